@@ -1,0 +1,303 @@
+/*
+ * sdemc.h -- C ABI of the MI355X (gfx950) Monte Carlo path-integration library.
+ *
+ * The library prices functionals of jump-diffusion and Levy-driven SDEs by
+ * Euler-Maruyama stepping, one path per GPU lane, with the path state held in
+ * registers and the estimator sums reduced in fp64.  It replaces the hot path
+ * of piers-hinds/sde_mc:
+ *
+ *   JumpDiffusionSolver.solve            sde_mc/solvers.py:164-226
+ *   JumpDiffusionSolver.multilevel_solve sde_mc/solvers.py:228-307
+ *   DiffusionSolver.solve / multilevel   sde_mc/solvers.py:68-119
+ *   EulerScheme.step                     sde_mc/schemes.py:5-13
+ *   Option.__call__ + discounter         sde_mc/options.py:167-176, 324-337
+ *   the (sum, sum of squares) reduction  sde_mc/mc.py:105-120, sde_mc/mlmc.py:45-69
+ *
+ * Every entry point returns an int status (SDEMC_OK == 0) and never throws.
+ * Arguments are validated on the host before anything is launched.  Device
+ * buffers are owned by the caller; the library allocates no device memory
+ * (sdemc_batch_plan_create allocates HIP graph objects, which
+ * sdemc_batch_plan_destroy releases).
+ * All pointers named *_dev are device pointers; `stream` is a hipStream_t
+ * (NULL = the default stream).
+ */
+#ifndef SDEMC_H
+#define SDEMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDEMC_ABI_VERSION 1
+#define SDEMC_MAX_DIM 4
+#define SDEMC_CHUNK 64 /* paths per reduction chunk (one wavefront) */
+
+/* ---- status codes ------------------------------------------------------ */
+enum sdemc_status {
+  SDEMC_OK = 0,
+  SDEMC_ERR_ARG = 1,         /* invalid argument (checked on the host)          */
+  SDEMC_ERR_UNSUPPORTED = 2, /* no kernel for this model / payoff / grid         */
+  SDEMC_ERR_LAUNCH = 3,      /* a HIP call failed (see sdemc_last_hip_error)     */
+  SDEMC_ERR_WORKSPACE = 4,   /* workspace too small                              */
+  SDEMC_ERR_OVERFLOW = 5,    /* a path needed more grid steps than max_steps     */
+  SDEMC_ERR_DEVICE = 6       /* no usable device                                 */
+};
+
+/* ---- model description --------------------------------------------------
+ * Coefficients are affine in the state, per component i < dim:
+ *   drift      a_i(x) = a0[i] + a1[i] * x_i
+ *   diffusion  b_i(x) = b0[i] + b1[i] * x_i   (driven by the correlated BM W)
+ *   second     c_i(x) = c0[i] + c1[i] * x_i   (driven by B, see `noise`)
+ *   jump       g_i(x) = j0[i] + j1[i] * x_i   (multiplies the jump mark J)
+ * One Euler step over dt with increments dW (= chol * z * sqrt(dt)) is
+ *   x_i <- x_i + a_i(x) dt + b_i(x) dW_i + c_i(x) dB_i,
+ * and a jump with mark J adds g_i(y) J with y the state before the step
+ * (exact_jumps == 0, solvers.py:216-217) or after it (exact_jumps == 1,
+ * solvers.py:214-215).  All jumps are common to every component
+ * (solvers.py:146-148).
+ */
+enum sdemc_noise {
+  SDEMC_NOISE_DIAG = 0,         /* 'diag':  dB = 0                             sde.py:24-25   */
+  SDEMC_NOISE_INDEP_COMMON = 1, /* 'indep' in the jump solver: dB_i = one scalar
+                                   N(0,dt) shared by all components
+                                   (small-jump Gaussian, solvers.py:198-201)          */
+  SDEMC_NOISE_INDEP_FULL = 2    /* 'indep' in the diffusion solver: B is a second
+                                   correlated BM independent of W (solvers.py:80-81)   */
+};
+
+enum sdemc_jump_law {
+  SDEMC_JUMP_LOGNORMAL = 0, /* J = exp(jp[0] + jp[1] * Z) - 1            sde.py:325-326 */
+  SDEMC_JUMP_LEVY = 1       /* J = InverseCdf(U + 2^-24/3)               levy.py:10-30, 85-87
+                               jp = {c_minus, c_plus, 1/mu, alpha, eps, lambda,
+                                     y1, y2, y3, eps^-alpha, -1/alpha}          */
+};
+
+typedef struct sdemc_model {
+  int32_t dim;         /* 1..SDEMC_MAX_DIM                                  */
+  int32_t noise;       /* enum sdemc_noise                                  */
+  int32_t jump_law;    /* enum sdemc_jump_law                               */
+  int32_t exact_jumps; /* 0: jump coefficient at pre-step state, 1: post    */
+  double rate;         /* jump intensity lambda (0 = no jumps)              */
+  double x0[SDEMC_MAX_DIM];
+  double a0[SDEMC_MAX_DIM], a1[SDEMC_MAX_DIM];
+  double b0[SDEMC_MAX_DIM], b1[SDEMC_MAX_DIM];
+  double c0[SDEMC_MAX_DIM], c1[SDEMC_MAX_DIM];
+  double j0[SDEMC_MAX_DIM], j1[SDEMC_MAX_DIM];
+  double chol[SDEMC_MAX_DIM * SDEMC_MAX_DIM]; /* lower Cholesky factor, row-major */
+  double jp[12];                              /* jump-law parameters            */
+} sdemc_model;
+
+/* ---- payoff (options.py:156-321) ---------------------------------------- */
+enum sdemc_payoff_kind {
+  SDEMC_PAYOFF_EURO_CALL = 0,  /* options.py:179-199 */
+  SDEMC_PAYOFF_EURO_PUT = 1,   /* options.py:202-222 */
+  SDEMC_PAYOFF_BINARY_AON = 2, /* options.py:225-244 */
+  SDEMC_PAYOFF_DIGITAL = 3,    /* options.py:275-284 */
+  SDEMC_PAYOFF_BASKET_ARITH = 4, /* options.py:247-259 */
+  SDEMC_PAYOFF_BASKET_GEO = 5,   /* options.py:247-259 */
+  SDEMC_PAYOFF_RAINBOW = 6,    /* call on max, options.py:262-272 */
+  SDEMC_PAYOFF_BEST_OF = 7,    /* max(max_i S_i, K), options.py:314-321 */
+  SDEMC_PAYOFF_IDENTITY = 8    /* x_0 itself (martingale checks)   */
+};
+
+typedef struct sdemc_payoff {
+  int32_t kind;    /* enum sdemc_payoff_kind                                   */
+  int32_t log;     /* Option.log: exponentiate the state first (options.py:168) */
+  double strike;
+  double scale;    /* Option.discount: spot multiplier (options.py:169)         */
+  double disc;     /* discounter(T) applied to the payoff (mc.py:93)            */
+} sdemc_payoff;
+
+/* ---- one pricing job ------------------------------------------------------ */
+enum sdemc_grid {
+  SDEMC_GRID_UNIFORM = 0, /* N steps of T/N, per-step Poisson jump counts        */
+  SDEMC_GRID_ADAPTED = 1  /* jump-adapted grid of solvers.py:182-225, fp32 clock  */
+};
+
+typedef struct sdemc_job {
+  sdemc_model model;
+  sdemc_payoff payoff;
+  double T;              /* time_interval                                         */
+  int32_t steps;         /* num_steps (the fine level when coarse_factor > 0)       */
+  int32_t grid;          /* enum sdemc_grid                                       */
+  int32_t coarse_factor; /* 0: single level; M >= 2: coupled fine/coarse pair with
+                            steps / M coarse steps (solvers.py:228-307)            */
+  int32_t max_steps;     /* adapted grid: cap on grid steps per path (the reference's
+                            storage bound N + max_jumps, solvers.py:133,170)       */
+  uint64_t n_paths;
+  uint64_t seed;         /* Philox key                                            */
+  uint64_t path_offset;  /* index of the first path in the Philox counter space   */
+} sdemc_job;
+
+/* ---- optional per-path outputs (plain entry points only) -----------------
+ * Layouts are structure-of-arrays with the path index fastest, so one
+ * wavefront's 64 lanes touch 256 contiguous bytes per row.
+ */
+typedef struct sdemc_path_out {
+  float* payoff_dev;    /* [P] discounted payoff (coupled: fine minus coarse)        */
+  float* terminal_dev;  /* [dim][P] state at T (coupled: [2][dim][P], fine first)    */
+  int32_t* nsteps_dev;  /* [P] grid steps taken (adapted grid)                      */
+  float* paths_dev;     /* [S+1][dim][P] states after each step, S = steps (uniform)
+                           or max_steps (adapted); rows past a path's end repeat
+                           its terminal state                                       */
+  float* times_dev;     /* adapted only, [S+1][P] grid times                         */
+  float* left_dev;      /* adapted only, [S+1][dim][P] pre-jump states              */
+  float* jumps_dev;     /* adapted only, [S+1][dim][P] applied jump marks           */
+  float* normals_dev;   /* [S][K][P] increments used (scaled and correlated);
+                           K = dim ('diag'), 2*dim ('indep': column-major pairs)    */
+} sdemc_path_out;
+
+/* ---- host-supplied random inputs (fixed-increment mode) ------------------
+ * Uniform grid:  dw     [N][K][P] increments, already scaled by sqrt(dt) and
+ *                       correlated (the reference's `normals`, solvers.py:206);
+ *                       K = dim ('diag'), dim + 1 ('indep' common: the last row
+ *                       is the shared small-jump increment), 2*dim ('indep' full)
+ *                counts [N][P] jumps arriving in step n
+ *                jump_off [P+1] CSR offsets into jump_size (int64)
+ *                jump_size marks J, path-major in arrival order
+ * Adapted grid:  dw     [S][K][P] standard normals z (scaled in the kernel)
+ *                jump_time [J][P] cumulative arrival times (fp32)
+ *                jump_size [J][P] marks J
+ * Coupled jobs use the fine grid's arrays.
+ */
+typedef struct sdemc_fixed_in {
+  const float* dw_dev;
+  const uint8_t* counts_dev;
+  const int64_t* jump_off_dev;
+  const float* jump_time_dev;
+  const float* jump_size_dev;
+  int64_t n_marks;   /* uniform: length of jump_size (reads outside it are
+                        refused in the kernel and reported as SDEMC_ERR_ARG) */
+  int32_t n_rows;    /* rows in dw (N or S; coupled adapted: S * M)          */
+  int32_t max_jumps; /* adapted: J                                            */
+} sdemc_fixed_in;
+
+/* ---- statistics ------------------------------------------------------------
+ * stats_dev receives 4 doubles per job: {sum, sum of squares, n, flags}
+ * where flags != 0 marks a path that overflowed max_steps.  The sums are taken
+ * in fp64 over fixed 64-path chunks and then over chunks in a fixed order, so
+ * a job's result depends only on the job, never on the launch it ran in.
+ */
+
+/* Library identity. */
+int sdemc_abi_version(void);
+const char* sdemc_status_string(int status);
+int sdemc_last_hip_error(void);
+size_t sdemc_sizeof_job(void);
+size_t sdemc_sizeof_model(void);
+size_t sdemc_sizeof_path_out(void);
+size_t sdemc_sizeof_fixed_in(void);
+
+/* Workspace (bytes) one plain launch needs: fp64 partials for every chunk. */
+size_t sdemc_workspace_bytes(uint64_t n_paths); /* 2 doubles + 1 int32 per chunk */
+
+/* Device properties: CU count, clock (kHz), wavefront size. */
+int sdemc_device_info(int device, int* cu_count, int* clock_khz, int* wave_size);
+
+/* Plain path, in-kernel Philox (RNG mode).  One launch for the paths, one for
+ * the fixed-order finalisation.  Replaces JumpDiffusionSolver.solve +
+ * payoff + the mc_simple sums (solvers.py:164-226, mc.py:87-117); with
+ * coarse_factor > 0 it replaces multilevel_solve + the MLMC level sums
+ * (solvers.py:228-307, mlmc.py:61-69).  `out` may be NULL. */
+int sdemc_simulate(const sdemc_job* job, double* stats_dev, const sdemc_path_out* out,
+                   void* workspace_dev, size_t workspace_bytes, int waves_per_block,
+                   void* stream);
+
+/* Plain path fed with host-generated increments, jump counts/times and marks
+ * instead of in-kernel draws (parity mode; the HBM-bound variant). */
+int sdemc_simulate_fixed(const sdemc_job* job, const sdemc_fixed_in* in, double* stats_dev,
+                         const sdemc_path_out* out, void* workspace_dev,
+                         size_t workspace_bytes, void* stream);
+
+/* Reduce chunk partials {sum, sumsq} of n_jobs jobs into stats (4 doubles each).
+ * chunk_base[j] is the first partial of job j, n_paths[j] its path count;
+ * chunk_flags holds one overflow flag per chunk; all arrays live on the device.  Replaces mc_estimates' inputs
+ * (helpers.py:51-68) with fp64 sums in a fixed order. */
+int sdemc_reduce_finalize(const double* partials_dev, const int32_t* chunk_flags_dev,
+                          const int64_t* chunk_base_dev, const uint64_t* n_paths_dev,
+                          int32_t n_jobs, double* stats_dev, void* stream);
+
+/* ---- batched persistent launch ------------------------------------------
+ * A job table of max_jobs slots lives on the device in the library's own
+ * prepared layout (sdemc_sizeof_prepared_job() bytes per slot), filled by
+ * the caller with bytes from sdemc_prepare_jobs.  A launch runs a tile list:
+ * tile t covers chunks [first, first + count) of slot `job`.  Persistent
+ * wavefronts take tiles from a device counter until the list is exhausted,
+ * so jobs of any size and any mix of models, grids and MLMC levels share
+ * one launch.  Partials of slot j live at partials[chunk_base_j + c].
+ * Tiles naming an unprepared slot or chunks outside a slot are skipped.
+ */
+typedef struct sdemc_tile {
+  int32_t job;   /* slot index                        */
+  int32_t first; /* first chunk                       */
+  int32_t count; /* chunks in this tile (>= 1)        */
+  int32_t cost;  /* scheduling weight (host use only) */
+} sdemc_tile;
+
+/* Bytes of one prepared job slot. */
+size_t sdemc_sizeof_prepared_job(void);
+
+/* Validate n host jobs and write their prepared slots (host memory,
+ * n * sdemc_sizeof_prepared_job() bytes).  chunk_base[i] is the first chunk
+ * partial of job i in the batch partials array.  *class_set receives the
+ * smallest kernel class set that covers the jobs (0: 1-D diag and 2-D
+ * 'indep' models; 1: every supported model). */
+int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_base,
+                       void* prepared_out, int32_t* class_set);
+
+typedef struct sdemc_batch_dev {
+  const void* jobs_dev;          /* [max_jobs] prepared slots                   */
+  const sdemc_tile* tiles_dev;   /* [max_tiles]                                 */
+  const int32_t* n_tiles_dev;    /* [1] tiles in this launch (read on device)   */
+  const int32_t* finalize_dev;   /* [max_jobs] nonzero: finalise slot this launch */
+  double* partials_dev;          /* [partial_chunks][2]                         */
+  int32_t* chunk_flags_dev;      /* [partial_chunks] overflow flags             */
+  double* stats_dev;             /* [max_jobs][4]                               */
+  int32_t* counter_dev;          /* [1] tile counter, zeroed by the launch      */
+  int64_t partial_chunks;        /* capacity of partials / chunk_flags          */
+  int32_t max_jobs;
+  int32_t max_tiles;
+  int32_t class_set;             /* from sdemc_prepare_jobs                     */
+  int32_t reserved;
+} sdemc_batch_dev;
+
+typedef struct sdemc_batch_tune {
+  int32_t waves_per_block; /* 1, 2, 4 or 8                   */
+  int32_t blocks_per_cu;   /* persistent workgroups per CU   */
+  int32_t reserved0;
+  int32_t reserved1;
+} sdemc_batch_tune;
+
+/* One batched launch: zero the tile counter, run the persistent tile kernel,
+ * finalise the flagged slots.  Everything is stream-ordered. */
+int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream);
+
+/* The same three launches captured once into a HIP graph and replayed by
+ * sdemc_batch_plan_launch; the graph reads n_tiles and the tables from the
+ * device, so one plan serves every launch that uses the same buffers. */
+typedef struct sdemc_batch_plan sdemc_batch_plan;
+int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tune,
+                            sdemc_batch_plan** plan);
+int sdemc_batch_plan_launch(sdemc_batch_plan* plan, void* stream);
+int sdemc_batch_plan_destroy(sdemc_batch_plan* plan);
+
+/* Philox4x32-10 block (Salmon et al., SC'11) evaluated on the device for
+ * n counters: out[4i..4i+3] = philox(ctr_i, key).  ctr is [n][4] uint32.
+ * Exposed for known-answer tests of the in-kernel generator. */
+int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint32_t* out_dev,
+                     int64_t n, void* stream);
+
+/* Standard normals exactly as the stepping kernels draw them: normal i of
+ * path p (i = step * K + k) for p in [0, n_paths), i in [0, n_per_path).
+ * out is [n_per_path][n_paths]. */
+int sdemc_normals_dev(uint64_t seed, uint64_t path_offset, int64_t n_paths, int32_t n_per_path,
+                      float* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDEMC_H */

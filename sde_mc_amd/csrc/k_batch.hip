@@ -1,0 +1,297 @@
+// k_batch.hip -- the batched persistent launch: one kernel runs a tile list
+// spanning any number of jobs (different models, grids, path counts, step
+// counts and MLMC levels).  Each wavefront repeatedly takes the next tile
+// from a device counter, so long and short tiles balance across the 256 CUs
+// without host involvement; the host orders tiles longest-first.
+#include <string.h>
+
+#include "sdemc_kernels.h"
+
+namespace sdemc {
+
+int validate_job(const sdemc_job* job);
+
+// One 64-path chunk of one job, RNG mode, no per-path outputs.  The result
+// equals the plain kernel's chunk: same Philox streams, same butterfly.
+template <int D, int FAM, int GRID, bool COUPLED>
+__device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, double* part,
+                                           int32_t* cf) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t P = (int64_t)job.n_paths;
+  const int64_t p = chunk * WAVE + lane;
+  const bool valid = p < P;
+  PathCtx c;
+  c.path = job.path_offset + (uint64_t)p;
+  c.p = p;
+  c.P = P;
+  c.k0 = (uint32_t)job.seed;
+  c.k1 = (uint32_t)(job.seed >> 32);
+  const StoreCtx st{};
+  float v = 0.0f;
+  int ovf = 0;
+  if (!COUPLED) {
+    float x[D];
+    if (GRID == SDEMC_GRID_UNIFORM) {
+      path_uniform_rng<D, FAM, false>(job, c, valid, x, st);
+    } else {
+      AdaptRngSrc<D, FAM> src;
+      src.init(job.m, c);
+      path_adapted<D, FAM, false>(job, src, valid, p, x, ovf, st);
+    }
+    if (valid) v = payoff_raw<D>(job.pay, x) * job.pay.disc;
+  } else {
+    float xf[D], xc[D];
+    if (GRID == SDEMC_GRID_UNIFORM) {
+      UniRngSrc<D, FAM> src;
+      src.init(job, c);
+      path_uniform_coupled<D, FAM>(job, src, xf, xc);
+    } else {
+      AdaptRngSrc<D, FAM> src;
+      src.init(job.m, c);
+      path_adapted_coupled<D, FAM>(job, src, valid, xf, xc, ovf);
+    }
+    if (valid) v = job.pay.disc * (payoff_raw<D>(job.pay, xf) - payoff_raw<D>(job.pay, xc));
+  }
+  const double dv = valid ? (double)v : 0.0;
+  chunk_store(dv, dv * dv, ovf, part, cf);
+}
+
+#define SDEMC_CASE(NOISE, D, FAM, GRID, C)                              \
+  case ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + C:                      \
+    chunk_body<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);           \
+    return true;
+
+#define SDEMC_CASES_DG(NOISE, D, FAM, GRID) \
+  SDEMC_CASE(NOISE, D, FAM, GRID, 0)        \
+  SDEMC_CASE(NOISE, D, FAM, GRID, 1)
+
+// Class set 0: the 1-D 'diag' family (Merton, GBM) and the 2-D 'indep'
+// family (Levy rainbow); set 1: every supported (noise, dim, grid, level).
+template <int SET>
+__device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double* part,
+                                          int32_t* cf) {
+  if (SET == 0) {
+    switch (job.class_key) {
+      SDEMC_CASES_DG(0, 1, FAM_MERTON, 0)
+      SDEMC_CASES_DG(0, 1, FAM_MERTON, 1)
+      SDEMC_CASES_DG(1, 2, FAM_LEVY, 0)
+      SDEMC_CASES_DG(1, 2, FAM_LEVY, 1)
+      default: return false;
+    }
+  } else {
+    switch (job.class_key) {
+      SDEMC_CASES_DG(0, 1, FAM_MERTON, 0)
+      SDEMC_CASES_DG(0, 1, FAM_MERTON, 1)
+      SDEMC_CASES_DG(0, 2, FAM_MERTON, 0)
+      SDEMC_CASES_DG(0, 2, FAM_MERTON, 1)
+      SDEMC_CASES_DG(0, 3, FAM_MERTON, 0)
+      SDEMC_CASES_DG(0, 3, FAM_MERTON, 1)
+      SDEMC_CASES_DG(0, 4, FAM_MERTON, 0)
+      SDEMC_CASES_DG(0, 4, FAM_MERTON, 1)
+      SDEMC_CASES_DG(1, 1, FAM_LEVY, 0)
+      SDEMC_CASES_DG(1, 1, FAM_LEVY, 1)
+      SDEMC_CASES_DG(1, 2, FAM_LEVY, 0)
+      SDEMC_CASES_DG(1, 2, FAM_LEVY, 1)
+      SDEMC_CASES_DG(1, 3, FAM_LEVY, 0)
+      SDEMC_CASES_DG(1, 3, FAM_LEVY, 1)
+      SDEMC_CASES_DG(1, 4, FAM_LEVY, 0)
+      SDEMC_CASES_DG(1, 4, FAM_LEVY, 1)
+      SDEMC_CASES_DG(2, 1, FAM_DOUBLE, 0)
+      SDEMC_CASES_DG(2, 2, FAM_DOUBLE, 0)
+      SDEMC_CASES_DG(2, 3, FAM_DOUBLE, 0)
+      SDEMC_CASES_DG(2, 4, FAM_DOUBLE, 0)
+      default: return false;
+    }
+  }
+}
+
+#undef SDEMC_CASE
+#undef SDEMC_CASES_DG
+
+template <int SET>
+__global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
+                                               const sdemc_tile* __restrict__ tiles,
+                                               const int32_t* __restrict__ n_tiles_dev,
+                                               double* __restrict__ partials,
+                                               int32_t* __restrict__ cflags,
+                                               int32_t* __restrict__ counter,
+                                               int64_t partial_chunks, int32_t max_jobs,
+                                               int32_t max_tiles) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  int32_t n_tiles = *n_tiles_dev;
+  n_tiles = n_tiles < max_tiles ? n_tiles : max_tiles;
+  for (;;) {
+    int32_t t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1);
+    t = __shfl(t, 0);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= n_tiles) break;
+    const sdemc_tile tile = tiles[t];
+    if (tile.job < 0 || tile.job >= max_jobs) continue;
+    const KJob& job = jobs[tile.job];
+    if (job.magic != KJOB_MAGIC) continue;
+    const int64_t nc = (int64_t)((job.n_paths + WAVE - 1) / WAVE);
+    int64_t c0 = tile.first < 0 ? 0 : tile.first;
+    int64_t c1 = (int64_t)tile.first + (tile.count > 0 ? tile.count : 0);
+    if (c1 > nc) c1 = nc;
+    for (int64_t chunk = c0; chunk < c1; ++chunk) {
+      const int64_t slot = job.chunk_base + chunk;
+      if (slot < 0 || slot >= partial_chunks) break;
+      run_chunk<SET>(job, chunk, partials + 2 * slot, cflags + slot);
+    }
+  }
+}
+
+__global__ void k_zero_counter(int32_t* counter) {
+  if (threadIdx.x == 0) counter[0] = 0;
+}
+
+// Fixed-order finalisation of the slots flagged this launch.
+
+__global__ __launch_bounds__(256) void k_finalize_slots(const KJob* __restrict__ jobs,
+                                                        const int32_t* __restrict__ mask,
+                                                        const double* partials,
+                                                        const int32_t* cflags,
+                                                        int64_t partial_chunks, double* stats) {
+  const int j = blockIdx.x;
+  if (mask[j] == 0) return;
+  const KJob& job = jobs[j];
+  if (job.magic != KJOB_MAGIC) return;
+  const int64_t nc = (int64_t)((job.n_paths + WAVE - 1) / WAVE);
+  if (job.chunk_base < 0 || job.chunk_base + nc > partial_chunks) return;
+  finalize_block(partials, cflags, job.chunk_base, nc, job.n_paths, stats + 4 * (int64_t)j);
+}
+
+static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {
+  if (!b || !tune) return SDEMC_ERR_ARG;
+  if (!b->jobs_dev || !b->tiles_dev || !b->n_tiles_dev || !b->finalize_dev || !b->partials_dev ||
+      !b->chunk_flags_dev || !b->stats_dev || !b->counter_dev)
+    return SDEMC_ERR_ARG;
+  if (b->max_jobs < 1 || b->max_tiles < 1 || b->partial_chunks < 0) return SDEMC_ERR_ARG;
+  if (b->class_set != 0 && b->class_set != 1) return SDEMC_ERR_ARG;
+  const int w = tune->waves_per_block;
+  if (w != 1 && w != 2 && w != 4 && w != 8) return SDEMC_ERR_ARG;
+  if (tune->blocks_per_cu < 1 || tune->blocks_per_cu > 64) return SDEMC_ERR_ARG;
+  return SDEMC_OK;
+}
+
+static int cu_count_cached() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    cus = prop.multiProcessorCount;
+  }
+  return cus;
+}
+
+static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, b.counter_dev);
+  const int wpb = tune.waves_per_block;
+  const unsigned grid = (unsigned)(cu_count_cached() * tune.blocks_per_cu);
+  const KJob* jobs = (const KJob*)b.jobs_dev;
+  if (b.class_set == 0) {
+    hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.tiles_dev,
+                       b.n_tiles_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
+                       b.partial_chunks, b.max_jobs, b.max_tiles);
+  } else {
+    hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.tiles_dev,
+                       b.n_tiles_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
+                       b.partial_chunks, b.max_jobs, b.max_tiles);
+  }
+  hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s, jobs,
+                     b.finalize_dev, b.partials_dev, b.chunk_flags_dev, b.partial_chunks,
+                     b.stats_dev);
+  return SDEMC_OK;
+}
+
+}  // namespace sdemc
+
+using namespace sdemc;
+
+struct sdemc_batch_plan {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+size_t sdemc_sizeof_prepared_job(void) { return sizeof(KJob); }
+
+int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_base,
+                       void* prepared_out, int32_t* class_set) {
+  if (n < 0 || (n > 0 && (!jobs || !chunk_base || !prepared_out))) return SDEMC_ERR_ARG;
+  int32_t set = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int st = validate_job(&jobs[i]);
+    if (st != SDEMC_OK) return st;
+    if (chunk_base[i] < 0) return SDEMC_ERR_ARG;
+    KJob k;
+    memset(&k, 0, sizeof(k));
+    make_kjob(jobs[i], k);
+    k.chunk_base = chunk_base[i];
+    const sdemc_model& m = jobs[i].model;
+    const bool low = (m.noise == SDEMC_NOISE_DIAG && m.dim == 1) ||
+                     (m.noise == SDEMC_NOISE_INDEP_COMMON && m.dim == 2);
+    if (!low) set = 1;
+    memcpy((char*)prepared_out + (size_t)i * sizeof(KJob), &k, sizeof(KJob));
+  }
+  if (class_set) *class_set = set;
+  return SDEMC_OK;
+}
+
+int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream) {
+  int st = check_batch(b, tune);
+  if (st != SDEMC_OK) return st;
+  enqueue_batch(*b, *tune, (hipStream_t)stream);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SDEMC_OK : SDEMC_ERR_LAUNCH;
+}
+
+int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tune,
+                            sdemc_batch_plan** plan) {
+  if (!plan) return SDEMC_ERR_ARG;
+  *plan = nullptr;
+  int st = check_batch(b, tune);
+  if (st != SDEMC_OK) return st;
+  (void)cu_count_cached();
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return SDEMC_ERR_LAUNCH;
+  sdemc_batch_plan* p = new sdemc_batch_plan();
+  p->graph = nullptr;
+  p->exec = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    enqueue_batch(*b, *tune, cs);
+    e = hipStreamEndCapture(cs, &p->graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    delete p;
+    return SDEMC_ERR_LAUNCH;
+  }
+  *plan = p;
+  return SDEMC_OK;
+}
+
+int sdemc_batch_plan_launch(sdemc_batch_plan* plan, void* stream) {
+  if (!plan || !plan->exec) return SDEMC_ERR_ARG;
+  const hipError_t e = hipGraphLaunch(plan->exec, (hipStream_t)stream);
+  return e == hipSuccess ? SDEMC_OK : SDEMC_ERR_LAUNCH;
+}
+
+int sdemc_batch_plan_destroy(sdemc_batch_plan* plan) {
+  if (!plan) return SDEMC_OK;
+  if (plan->exec) hipGraphExecDestroy(plan->exec);
+  if (plan->graph) hipGraphDestroy(plan->graph);
+  delete plan;
+  return SDEMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,289 @@
+// sdemc_api.hip -- extern "C" entry points of libsdemc: host-side validation,
+// dispatch to the family launchers, the fixed-order finalisation kernels and
+// the generator test kernels.  Nothing here throws across the ABI.
+#include <math.h>
+#include <string.h>
+
+#include "sdemc_kernels.h"
+
+namespace sdemc {
+
+static thread_local int g_last_hip_error = 0;
+
+static int hip_status(hipError_t e) {
+  if (e == hipSuccess) return SDEMC_OK;
+  g_last_hip_error = (int)e;
+  return SDEMC_ERR_LAUNCH;
+}
+
+static bool finite(double v) { return isfinite(v) != 0; }
+
+// Host-side checks of one job; everything the kernels assume is enforced
+// here (dimensions, grid sizes, jump law constants, payoff kind).
+int validate_job(const sdemc_job* job) {
+  if (!job) return SDEMC_ERR_ARG;
+  const sdemc_model& m = job->model;
+  if (m.dim < 1 || m.dim > SDEMC_MAX_DIM) return SDEMC_ERR_UNSUPPORTED;
+  if (m.noise < SDEMC_NOISE_DIAG || m.noise > SDEMC_NOISE_INDEP_FULL) return SDEMC_ERR_UNSUPPORTED;
+  if (m.jump_law != SDEMC_JUMP_LOGNORMAL && m.jump_law != SDEMC_JUMP_LEVY)
+    return SDEMC_ERR_UNSUPPORTED;
+  if (m.exact_jumps != 0 && m.exact_jumps != 1) return SDEMC_ERR_ARG;
+  if (!finite(m.rate) || m.rate < 0.0) return SDEMC_ERR_ARG;
+  for (int i = 0; i < m.dim; ++i) {
+    if (!finite(m.x0[i]) || !finite(m.a0[i]) || !finite(m.a1[i]) || !finite(m.b0[i]) ||
+        !finite(m.b1[i]) || !finite(m.c0[i]) || !finite(m.c1[i]) || !finite(m.j0[i]) ||
+        !finite(m.j1[i]))
+      return SDEMC_ERR_ARG;
+    for (int j = 0; j < m.dim; ++j)
+      if (!finite(m.chol[i * SDEMC_MAX_DIM + j])) return SDEMC_ERR_ARG;
+  }
+  if (m.rate > 0.0) {
+    if (m.jump_law == SDEMC_JUMP_LOGNORMAL) {
+      if (!finite(m.jp[0]) || !finite(m.jp[1]) || m.jp[1] < 0.0) return SDEMC_ERR_ARG;
+    } else {
+      const double cm = m.jp[0], cp = m.jp[1], imu = m.jp[2], al = m.jp[3], eps = m.jp[4];
+      const double lda = m.jp[5];
+      if (!(cm > 0.0) || !(cp > 0.0) || !finite(cm) || !finite(cp)) return SDEMC_ERR_ARG;
+      if (!finite(imu) || imu < 0.0) return SDEMC_ERR_ARG;
+      if (!(al > 0.0 && al < 2.0) || !(eps > 0.0 && eps < 1.0)) return SDEMC_ERR_ARG;
+      if (!(lda > 0.0) || !finite(lda)) return SDEMC_ERR_ARG;
+      for (int k = 6; k <= 10; ++k)
+        if (!finite(m.jp[k])) return SDEMC_ERR_ARG;
+      if (!(m.jp[6] <= m.jp[7] && m.jp[7] <= m.jp[8])) return SDEMC_ERR_ARG;
+    }
+  }
+  const sdemc_payoff& p = job->payoff;
+  if (p.kind < SDEMC_PAYOFF_EURO_CALL || p.kind > SDEMC_PAYOFF_IDENTITY) return SDEMC_ERR_UNSUPPORTED;
+  if (p.log != 0 && p.log != 1) return SDEMC_ERR_ARG;
+  if (!finite(p.strike) || !finite(p.scale) || !finite(p.disc)) return SDEMC_ERR_ARG;
+  if (!finite(job->T) || !(job->T > 0.0)) return SDEMC_ERR_ARG;
+  if (job->steps < 1) return SDEMC_ERR_ARG;
+  if (job->grid != SDEMC_GRID_UNIFORM && job->grid != SDEMC_GRID_ADAPTED) return SDEMC_ERR_ARG;
+  if (job->coarse_factor != 0) {
+    if (job->coarse_factor < 2) return SDEMC_ERR_ARG;
+    if (job->steps % job->coarse_factor != 0) return SDEMC_ERR_ARG;
+  }
+  if (job->grid == SDEMC_GRID_ADAPTED && job->max_steps < 1) return SDEMC_ERR_ARG;
+  if (job->n_paths > (1ull << 37)) return SDEMC_ERR_ARG;
+  // bounded expected jumps per path keeps every per-step jump loop short
+  if (m.rate * job->T > 1.0e5) return SDEMC_ERR_ARG;
+  if (m.noise == SDEMC_NOISE_INDEP_FULL && job->grid == SDEMC_GRID_ADAPTED)
+    return SDEMC_ERR_UNSUPPORTED;
+  return SDEMC_OK;
+}
+
+__host__ __device__ int64_t n_chunks(uint64_t n_paths) { return (int64_t)((n_paths + WAVE - 1) / WAVE); }
+
+size_t workspace_bytes(uint64_t n_paths) {
+  const size_t nc = (size_t)n_chunks(n_paths);
+  size_t b = nc * 2 * sizeof(double) + nc * sizeof(int32_t);
+  return (b + 255) & ~(size_t)255;
+}
+
+__global__ __launch_bounds__(256) void k_finalize_one(const double* partials,
+                                                      const int32_t* cflags, int64_t nc,
+                                                      uint64_t n, double* stats) {
+  finalize_block(partials, cflags, 0, nc, n, stats);
+}
+
+__global__ __launch_bounds__(256) void k_finalize(const double* partials, const int32_t* cflags,
+                                                  const int64_t* chunk_base,
+                                                  const uint64_t* n_paths, const int32_t* mask,
+                                                  int32_t n_jobs, double* stats) {
+  const int j = blockIdx.x;
+  if (j >= n_jobs) return;
+  if (mask && mask[j] == 0) return;
+  const uint64_t n = n_paths[j];
+  finalize_block(partials, cflags, chunk_base[j], n_chunks(n), n, stats + 4 * (int64_t)j);
+}
+
+__global__ void k_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const U4 r = philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], k0, k1);
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+__global__ void k_normals(uint32_t k0, uint32_t k1, uint64_t path_offset, int64_t n_paths,
+                          int32_t n_per_path, float* out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_paths) return;
+  NormalStream ns;
+  ns.init(k0, k1, path_offset + (uint64_t)p);
+  for (int32_t i = 0; i < n_per_path; ++i) out[(int64_t)i * n_paths + p] = ns.get((uint32_t)i);
+}
+
+}  // namespace sdemc
+
+using namespace sdemc;
+
+extern "C" {
+
+int sdemc_abi_version(void) { return SDEMC_ABI_VERSION; }
+
+const char* sdemc_status_string(int status) {
+  switch (status) {
+    case SDEMC_OK: return "ok";
+    case SDEMC_ERR_ARG: return "invalid argument";
+    case SDEMC_ERR_UNSUPPORTED: return "unsupported model, payoff or grid";
+    case SDEMC_ERR_LAUNCH: return "HIP launch or runtime error";
+    case SDEMC_ERR_WORKSPACE: return "workspace too small";
+    case SDEMC_ERR_OVERFLOW: return "a path exceeded max_steps";
+    case SDEMC_ERR_DEVICE: return "no usable device";
+    default: return "unknown status";
+  }
+}
+
+int sdemc_last_hip_error(void) { return g_last_hip_error; }
+size_t sdemc_sizeof_job(void) { return sizeof(sdemc_job); }
+size_t sdemc_sizeof_model(void) { return sizeof(sdemc_model); }
+size_t sdemc_sizeof_path_out(void) { return sizeof(sdemc_path_out); }
+size_t sdemc_sizeof_fixed_in(void) { return sizeof(sdemc_fixed_in); }
+
+size_t sdemc_workspace_bytes(uint64_t n_paths) { return workspace_bytes(n_paths); }
+
+int sdemc_device_info(int device, int* cu_count, int* clock_khz, int* wave_size) {
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    g_last_hip_error = (int)e;
+    return SDEMC_ERR_DEVICE;
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  if (wave_size) *wave_size = prop.warpSize;
+  return SDEMC_OK;
+}
+
+static int dispatch_rng(const sdemc_job& job, const LaunchArgs& a) {
+  switch (job.model.noise) {
+    case SDEMC_NOISE_DIAG: return launch_rng_merton(job, a);
+    case SDEMC_NOISE_INDEP_COMMON: return launch_rng_levy(job, a);
+    case SDEMC_NOISE_INDEP_FULL: return launch_rng_double(job, a);
+    default: return SDEMC_ERR_UNSUPPORTED;
+  }
+}
+
+static int dispatch_fixed(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a) {
+  switch (job.model.noise) {
+    case SDEMC_NOISE_DIAG: return launch_fixed_merton(job, in, a);
+    case SDEMC_NOISE_INDEP_COMMON: return launch_fixed_levy(job, in, a);
+    case SDEMC_NOISE_INDEP_FULL: return launch_fixed_double(job, in, a);
+    default: return SDEMC_ERR_UNSUPPORTED;
+  }
+}
+
+static int finalize_plain(const sdemc_job& job, double* partials, int32_t* cflags,
+                          double* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_one, dim3(1), dim3(256), 0, s, partials, cflags,
+                     n_chunks(job.n_paths), job.n_paths, stats);
+  return hip_status(hipGetLastError());
+}
+
+int sdemc_simulate(const sdemc_job* job, double* stats_dev, const sdemc_path_out* out,
+                   void* workspace_dev, size_t workspace_bytes_, int waves_per_block,
+                   void* stream) {
+  int st = validate_job(job);
+  if (st != SDEMC_OK) return st;
+  if (!stats_dev || !workspace_dev) return SDEMC_ERR_ARG;
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
+      waves_per_block != 8)
+    return SDEMC_ERR_ARG;
+  if (workspace_bytes_ < workspace_bytes(job->n_paths)) return SDEMC_ERR_WORKSPACE;
+  const int64_t nc = n_chunks(job->n_paths);
+  LaunchArgs a;
+  a.partials = (double*)workspace_dev;
+  a.cflags = (int32_t*)((char*)workspace_dev + (size_t)nc * 2 * sizeof(double));
+  memset(&a.out, 0, sizeof(a.out));
+  if (out) a.out = *out;
+  a.store = a.out.paths_dev != nullptr;
+  a.wpb = waves_per_block;
+  a.stream = (hipStream_t)stream;
+  if (a.store && job->coarse_factor > 0) return SDEMC_ERR_UNSUPPORTED;
+  if (nc > 0) {
+    st = dispatch_rng(*job, a);
+    if (st != SDEMC_OK) return st;
+  }
+  return finalize_plain(*job, a.partials, a.cflags, stats_dev, a.stream);
+}
+
+int sdemc_simulate_fixed(const sdemc_job* job, const sdemc_fixed_in* in, double* stats_dev,
+                         const sdemc_path_out* out, void* workspace_dev,
+                         size_t workspace_bytes_, void* stream) {
+  int st = validate_job(job);
+  if (st != SDEMC_OK) return st;
+  if (!in || !stats_dev || !workspace_dev || !in->dw_dev) return SDEMC_ERR_ARG;
+  if (workspace_bytes_ < workspace_bytes(job->n_paths)) return SDEMC_ERR_WORKSPACE;
+  const int M = job->coarse_factor;
+  if (job->grid == SDEMC_GRID_UNIFORM) {
+    if (in->n_rows < job->steps) return SDEMC_ERR_ARG;
+    if ((in->counts_dev == nullptr) != (in->jump_off_dev == nullptr)) return SDEMC_ERR_ARG;
+    if (in->n_marks < 0) return SDEMC_ERR_ARG;
+    if (in->n_marks > 0 && !in->jump_size_dev) return SDEMC_ERR_ARG;
+  } else {
+    const int64_t need = (int64_t)job->max_steps * (M > 0 ? M : 1);
+    if ((int64_t)in->n_rows < need) return SDEMC_ERR_ARG;
+    if (in->max_jumps < 0) return SDEMC_ERR_ARG;
+    if (in->max_jumps > 0 && (!in->jump_time_dev || !in->jump_size_dev)) return SDEMC_ERR_ARG;
+  }
+  const int64_t nc = n_chunks(job->n_paths);
+  LaunchArgs a;
+  a.partials = (double*)workspace_dev;
+  a.cflags = (int32_t*)((char*)workspace_dev + (size_t)nc * 2 * sizeof(double));
+  memset(&a.out, 0, sizeof(a.out));
+  if (out) {
+    a.out.payoff_dev = out->payoff_dev;
+    a.out.terminal_dev = out->terminal_dev;
+    a.out.nsteps_dev = out->nsteps_dev;
+  }
+  a.store = false;
+  a.wpb = 4;
+  a.stream = (hipStream_t)stream;
+  if (nc > 0) {
+    st = dispatch_fixed(*job, *in, a);
+    if (st != SDEMC_OK) return st;
+  }
+  return finalize_plain(*job, a.partials, a.cflags, stats_dev, a.stream);
+}
+
+int sdemc_reduce_finalize(const double* partials_dev, const int32_t* chunk_flags_dev,
+                          const int64_t* chunk_base_dev, const uint64_t* n_paths_dev,
+                          int32_t n_jobs, double* stats_dev, void* stream) {
+  if (n_jobs < 0) return SDEMC_ERR_ARG;
+  if (n_jobs == 0) return SDEMC_OK;
+  if (!partials_dev || !chunk_flags_dev || !chunk_base_dev || !n_paths_dev || !stats_dev)
+    return SDEMC_ERR_ARG;
+  hipLaunchKernelGGL(k_finalize, dim3((unsigned)n_jobs), dim3(256), 0, (hipStream_t)stream,
+                     partials_dev, chunk_flags_dev, chunk_base_dev, n_paths_dev,
+                     (const int32_t*)nullptr, n_jobs, stats_dev);
+  return hip_status(hipGetLastError());
+}
+
+int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint32_t* out_dev,
+                     int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && (!ctr_dev || !out_dev))) return SDEMC_ERR_ARG;
+  if (n == 0) return SDEMC_OK;
+  const int64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffLL) return SDEMC_ERR_ARG;
+  hipLaunchKernelGGL(k_philox, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, ctr_dev,
+                     key0, key1, out_dev, n);
+  return hip_status(hipGetLastError());
+}
+
+int sdemc_normals_dev(uint64_t seed, uint64_t path_offset, int64_t n_paths, int32_t n_per_path,
+                      float* out_dev, void* stream) {
+  if (n_paths < 0 || n_per_path < 0) return SDEMC_ERR_ARG;
+  if (n_paths == 0 || n_per_path == 0) return SDEMC_OK;
+  if (!out_dev) return SDEMC_ERR_ARG;
+  const int64_t blocks = (n_paths + 255) / 256;
+  if (blocks > 0x7fffffffLL) return SDEMC_ERR_ARG;
+  hipLaunchKernelGGL(k_normals, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (uint32_t)seed, (uint32_t)(seed >> 32), path_offset, n_paths, n_per_path,
+                     out_dev);
+  return hip_status(hipGetLastError());
+}
+
+}  // extern "C"

@@ -1,0 +1,1062 @@
+// sdemc_core.h -- device code of the gfx950 path-integration kernels.
+//
+// One Monte Carlo path per lane of a 64-wide wavefront.  The path state, the
+// Philox counters and the pending jump live in registers for the whole
+// horizon; nothing per step touches HBM in RNG mode.  A wavefront's 64 paths
+// form one reduction chunk whose fp64 (sum, sum of squares) is reduced by a
+// fixed butterfly of lane shuffles, so a job's result does not depend on which
+// launch, workgroup or tile computed it.
+//
+// Reference semantics restated here (file:line in piers-hinds/sde_mc):
+//   Euler step                      sde_mc/schemes.py:5-10
+//   correlated increments L z sqrt(dt)   sde_mc/solvers.py:51-56, 195-201
+//   jump-adapted grid and clock     sde_mc/solvers.py:182-225
+//   coupled fine/coarse levels      sde_mc/solvers.py:228-307
+//   lognormal marks                 sde_mc/sde.py:325-326
+//   Levy marks (inverse CDF)        sde_mc/levy.py:7-30, 85-87
+//   payoffs and discounting         sde_mc/options.py:167-176, 179-321, 334-337
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sdemc.h"
+
+namespace sdemc {
+
+enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2 };
+enum { STREAM_BM = 0, STREAM_JUMP = 1, STREAM_TEST = 7 };
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel Random Numbers: As Easy
+// as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies, key schedule by
+// the Weyl constants.  Counter = (block, stream, path_lo, path_hi), key = seed.
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                      uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// Uniform in (0,1): odd multiples of 2^-24, exact in fp32 (and in fp64).
+__device__ __forceinline__ float u01_open(uint32_t w) {
+  return (float)((w >> 8) | 1u) * 0x1p-24f;
+}
+// Uniform in [0,1): multiples of 2^-24, exact in fp32.
+__device__ __forceinline__ float u01_half(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
+
+// Box-Muller on the hardware transcendental units: v_log_f32 is log2,
+// v_sin_f32 / v_cos_f32 take the angle in revolutions, so 2*pi*a needs no
+// range reduction.  z0 = r cos(2 pi a), z1 = r sin(2 pi a), r = sqrt(-2 ln u).
+__device__ __forceinline__ void box_muller(uint32_t w0, uint32_t w1, float& z0, float& z1) {
+  const float u = u01_open(w0);
+  const float a = u01_half(w1);
+  const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u));
+  z0 = r * __builtin_amdgcn_cosf(a);
+  z1 = r * __builtin_amdgcn_sinf(a);
+}
+
+__device__ __forceinline__ float fast_exp(float v) {
+  return __builtin_amdgcn_exp2f(v * 1.44269504088896341f);
+}
+__device__ __forceinline__ float fast_pow(float b, float e) {  // b > 0
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b));
+}
+__device__ __forceinline__ float fast_ln(float v) {
+  return 0.693147180559945309f * __builtin_amdgcn_logf(v);
+}
+
+// Normal number i of one path (i = step * K + k): word (i mod 4) of Philox
+// block i / 4 on the Brownian stream, paired (0,1) and (2,3) by Box-Muller.
+// The last block is cached, so sequential reads cost one Philox per 4 normals.
+struct NormalStream {
+  uint32_t k0, k1, plo, phi;
+  uint32_t blk;
+  float z0, z1, z2, z3;
+  __device__ __forceinline__ void init(uint32_t key0, uint32_t key1, uint64_t path) {
+    k0 = key0;
+    k1 = key1;
+    plo = (uint32_t)path;
+    phi = (uint32_t)(path >> 32);
+    blk = 0xffffffffu;
+  }
+  __device__ __forceinline__ float get(uint32_t i) {
+    const uint32_t b = i >> 2;
+    if (b != blk) {
+      blk = b;
+      const U4 w = philox4x32_10(b, STREAM_BM, plo, phi, k0, k1);
+      box_muller(w.x, w.y, z0, z1);
+      box_muller(w.z, w.w, z2, z3);
+    }
+    const uint32_t s = i & 3u;
+    return s == 0 ? z0 : (s == 1 ? z1 : (s == 2 ? z2 : z3));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Model constants as the kernels read them: fp32 coefficients and fp64 jump
+// clock constants, converted once on the host so every lane reads them as
+// wave-uniform scalars from the kernel arguments (or the job table).
+// ---------------------------------------------------------------------------
+struct KModel {
+  float x0[SDEMC_MAX_DIM], a0[SDEMC_MAX_DIM], a1[SDEMC_MAX_DIM];
+  float b0[SDEMC_MAX_DIM], b1[SDEMC_MAX_DIM], c0[SDEMC_MAX_DIM], c1[SDEMC_MAX_DIM];
+  float j0[SDEMC_MAX_DIM], j1[SDEMC_MAX_DIM];
+  float L[SDEMC_MAX_DIM][SDEMC_MAX_DIM];
+  float jm, js;  // lognormal: mean and sd of the log-jump
+  float lv[11];  // Levy inverse-CDF constants, see levy_icdf
+  int32_t exact, law, dim, noise;
+  double rate, inv_rate;
+};
+
+struct KPayoff {
+  int32_t kind, log;
+  float strike, scale, disc;
+};
+
+// Everything one job needs in the kernels, plus derived grid constants.
+struct KJob {
+  KModel m;
+  KPayoff pay;
+  double T;
+  double n_over_T;  // steps / T (fp64): jump arrival -> step index
+  float h;          // fp32(T / steps): fine step (reference: torch.tensor(T / N))
+  float sqh;        // sqrt(h)
+  float hc;         // coupled uniform grid: fp32(T * M / steps)
+  float sqhc;
+  int32_t steps, grid, M, max_steps;
+  uint64_t n_paths, seed, path_offset;
+  int64_t chunk_base;  // first chunk partial of this job (batched launches)
+  uint32_t magic;      // KJOB_MAGIC once prepared on the host
+  int32_t class_key;   // batched-kernel dispatch key, see class_key_of
+};
+
+constexpr uint32_t KJOB_MAGIC = 0x5DE3C0DEu;
+
+// Dispatch key of a prepared job: ((noise * 4 + dim - 1) * 2 + grid) * 2 + coupled.
+inline int32_t class_key_of(int noise, int dim, int grid, bool coupled) {
+  return ((noise * 4 + (dim - 1)) * 2 + grid) * 2 + (coupled ? 1 : 0);
+}
+
+inline void make_kjob(const sdemc_job& j, KJob& k) {
+  const sdemc_model& m = j.model;
+  KModel& f = k.m;
+  for (int i = 0; i < SDEMC_MAX_DIM; ++i) {
+    f.x0[i] = (float)m.x0[i];
+    f.a0[i] = (float)m.a0[i];
+    f.a1[i] = (float)m.a1[i];
+    f.b0[i] = (float)m.b0[i];
+    f.b1[i] = (float)m.b1[i];
+    f.c0[i] = (float)m.c0[i];
+    f.c1[i] = (float)m.c1[i];
+    f.j0[i] = (float)m.j0[i];
+    f.j1[i] = (float)m.j1[i];
+    for (int q = 0; q < SDEMC_MAX_DIM; ++q) f.L[i][q] = (float)m.chol[i * SDEMC_MAX_DIM + q];
+  }
+  f.exact = m.exact_jumps;
+  f.law = m.jump_law;
+  f.dim = m.dim;
+  f.noise = m.noise;
+  f.jm = (float)m.jp[0];
+  f.js = (float)m.jp[1];
+  for (int q = 0; q < 11; ++q) f.lv[q] = 0.0f;
+  if (m.jump_law == SDEMC_JUMP_LEVY) {
+    // jp = {cm, cp, 1/mu, alpha, eps, lambda, y1, y2, y3, eps^-alpha, -1/alpha}
+    const double cm = m.jp[0], cp = m.jp[1], imu = m.jp[2], al = m.jp[3];
+    const double lda = m.jp[5], epa = m.jp[9];
+    f.lv[0] = (float)m.jp[6];
+    f.lv[1] = (float)m.jp[7];
+    f.lv[2] = (float)m.jp[8];
+    f.lv[3] = (float)(imu > 0.0 ? lda / (cm * imu) : 0.0);  // mu*lda/cm
+    f.lv[4] = (float)imu;                                   // 1/mu
+    f.lv[5] = (float)(al * lda / cm);                       // alpha*lda/cm
+    f.lv[6] = (float)(1.0 - al * imu);                      // 1 - alpha/mu
+    f.lv[7] = (float)(-al / cp * lda);                      // -alpha*lda/cp
+    f.lv[8] = (float)((-al / cp) * (-cm * imu - cm * (epa - 1.0) / al) + epa);
+    f.lv[9] = (float)(imu > 0.0 ? lda / (cp * imu) : 0.0);  // mu*lda/cp
+    f.lv[10] = (float)m.jp[10];                             // -1/alpha
+  }
+  f.rate = m.rate;
+  f.inv_rate = m.rate > 0.0 ? 1.0 / m.rate : 0.0;
+  k.pay.kind = j.payoff.kind;
+  k.pay.log = j.payoff.log;
+  k.pay.strike = (float)j.payoff.strike;
+  k.pay.scale = (float)j.payoff.scale;
+  k.pay.disc = (float)j.payoff.disc;
+  k.T = j.T;
+  k.steps = j.steps;
+  k.grid = j.grid;
+  k.M = j.coarse_factor;
+  k.max_steps = j.max_steps;
+  k.n_over_T = (double)j.steps / j.T;
+  k.h = (float)(j.T / (double)j.steps);
+  k.sqh = sqrtf(k.h);
+  if (j.coarse_factor > 0) {
+    k.hc = (float)(j.T * (double)j.coarse_factor / (double)j.steps);
+    k.sqhc = sqrtf(k.hc);
+  } else {
+    k.hc = k.h;
+    k.sqhc = k.sqh;
+  }
+  k.n_paths = j.n_paths;
+  k.seed = j.seed;
+  k.path_offset = j.path_offset;
+  k.chunk_base = 0;
+  k.magic = KJOB_MAGIC;
+  k.class_key = class_key_of(m.noise, m.dim, j.grid, j.coarse_factor > 0);
+}
+
+// Inverse CDF of the Levy jump law of levy.py:22-30 on y in (0,1):
+//   y <= y1 : 1 - ... tail branch  x1 = ln(mu*lda*y/cm)/mu - 1
+//   y <  y2 : x2 = -(alpha*(lda*y/cm - 1/mu) + 1)^(-1/alpha)
+//   y <  y3 : x3 = ((-alpha/cp)*(lda*y - cm/mu - cm*(eps^-a - 1)/a) + eps^-a)^(-1/a)
+//   else    : x4 = 1 - ln(mu*lda*(1-y)/cp)/mu
+// Branch selection compares the fp32 uniform with fp32 thresholds, as the
+// reference's torch.where on fp32 tensors does.
+__device__ __forceinline__ float levy_icdf(const KModel& f, float y) {
+  float x;
+  if (y <= f.lv[0]) {
+    x = fast_ln(f.lv[3] * y) * f.lv[4] - 1.0f;
+  } else if (y < f.lv[1]) {
+    x = -fast_pow(f.lv[5] * y + f.lv[6], f.lv[10]);
+  } else if (y < f.lv[2]) {
+    x = fast_pow(f.lv[7] * y + f.lv[8], f.lv[10]);
+  } else {
+    x = 1.0f - f.lv[4] * fast_ln(f.lv[9] * (1.0f - y));
+  }
+  return x;
+}
+
+// Jump mark from two Philox words.
+__device__ __forceinline__ float jump_mark(const KModel& f, uint32_t m1, uint32_t m2) {
+  if (f.law == SDEMC_JUMP_LEVY) {
+    // torch.rand + UNIFORM_TOL / 3 (levy.py:7, 86), in fp32
+    const float y = u01_half(m1) + 0x1.555556p-26f;
+    return levy_icdf(f, y);
+  }
+  const float u = u01_open(m1);
+  const float a = u01_half(m2);
+  const float z = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u)) *
+                  __builtin_amdgcn_cosf(a);
+  return fast_exp(f.jm + f.js * z) - 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Payoff on the terminal state (options.py:167-176): spot = scale * (exp(x)
+// if log else x), payoff(spot) * disc.
+// ---------------------------------------------------------------------------
+// Undiscounted payoff; the caller applies the discount factor, as
+// mc.py:93 and mlmc.py:64-65 multiply after the payoff.
+template <int D>
+__device__ __forceinline__ float payoff_raw(const KPayoff& p, const float (&x)[D]) {
+  float s[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) s[i] = p.scale * (p.log ? fast_exp(x[i]) : x[i]);
+  const float K = p.strike;
+  float v;
+  switch (p.kind) {
+    case SDEMC_PAYOFF_EURO_CALL: v = s[0] > K ? s[0] - K : 0.0f; break;
+    case SDEMC_PAYOFF_EURO_PUT: v = s[0] < K ? K - s[0] : 0.0f; break;
+    case SDEMC_PAYOFF_BINARY_AON: v = s[0] >= K ? s[0] : 0.0f; break;
+    case SDEMC_PAYOFF_DIGITAL: v = s[0] > K ? 1.0f : 0.0f; break;
+    case SDEMC_PAYOFF_BASKET_ARITH: {
+      float m = 0.0f;
+#pragma unroll
+      for (int i = 0; i < D; ++i) m += s[i];
+      m = m / (float)D;
+      v = m > K ? m - K : 0.0f;
+      break;
+    }
+    case SDEMC_PAYOFF_BASKET_GEO: {
+      float m = 0.0f;
+#pragma unroll
+      for (int i = 0; i < D; ++i) m += fast_ln(s[i]);
+      m = fast_exp(m / (float)D);
+      v = m > K ? m - K : 0.0f;
+      break;
+    }
+    case SDEMC_PAYOFF_RAINBOW: {
+      float m = s[0];
+#pragma unroll
+      for (int i = 1; i < D; ++i) m = fmaxf(m, s[i]);
+      v = m > K ? m - K : 0.0f;
+      break;
+    }
+    case SDEMC_PAYOFF_BEST_OF: {
+      float m = s[0];
+#pragma unroll
+      for (int i = 1; i < D; ++i) m = fmaxf(m, s[i]);
+      v = fmaxf(m, K);
+      break;
+    }
+    default: v = s[0]; break;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Euler step (schemes.py:5-10) for one component and the jump update.
+// ---------------------------------------------------------------------------
+template <int D, int FAM>
+__device__ __forceinline__ void euler(const KModel& f, float (&x)[D], float dt,
+                                      const float (&dW)[D], const float (&dB)[D]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const float a = f.a0[i] + f.a1[i] * x[i];
+    const float b = f.b0[i] + f.b1[i] * x[i];
+    float diff = b * dW[i];
+    if (FAM != FAM_MERTON) diff += (f.c0[i] + f.c1[i] * x[i]) * dB[i];
+    x[i] = x[i] + a * dt + diff;
+  }
+}
+
+// x += g(y) * J with g(y) = j0 + j1 * y (sde.py:374-375, levy.py:119-120).
+template <int D>
+__device__ __forceinline__ void add_jump(const KModel& f, float (&x)[D], const float (&y)[D],
+                                         float J) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) x[i] = x[i] + (f.j0[i] + f.j1[i] * y[i]) * J;
+}
+
+// Number of normals drawn per step.
+template <int D, int FAM>
+struct NK {
+  static constexpr int K = FAM == FAM_MERTON ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+};
+
+// Correlated, scaled increments from K standard normals:
+//   dW = L (z[0:D] sqrt(dt)),  dB = z[D] sqrt(dt) (common) or L (z[D:2D] sqrt(dt)).
+template <int D, int FAM>
+__device__ __forceinline__ void correlate(const KModel& f, const float* z, float sq,
+                                          float (&dW)[D], float (&dB)[D]) {
+  float zs[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) zs[j] = z[j] * sq;
+  if (D == 1) {
+    dW[0] = zs[0];
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int j = 0; j <= i; ++j) acc += f.L[i][j] * zs[j];
+      dW[i] = acc;
+    }
+  }
+  if (FAM == FAM_LEVY) {
+    const float c = z[D] * sq;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dB[i] = c;
+  } else if (FAM == FAM_DOUBLE) {
+    float zb[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) zb[j] = z[D + j] * sq;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int j = 0; j <= i; ++j) acc += f.L[i][j] * zb[j];
+      dB[i] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) dB[i] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jump arrivals drawn in the kernel.  Jump j (j = 0, 1, ...) takes Philox
+// block j of the jump stream: word x gives the exponential gap to it,
+// words y, z its mark.  Arrival times accumulate in fp64.
+// ---------------------------------------------------------------------------
+struct JumpClock {
+  double tau;   // arrival time of jump j
+  uint32_t j;   // index of the pending jump
+  uint32_t m1, m2;
+  __device__ __forceinline__ void draw(uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi,
+                                       double inv_rate) {
+    const U4 w = philox4x32_10(j, STREAM_JUMP, plo, phi, k0, k1);
+    const double u = ((double)w.x + 0.5) * 0x1p-32;
+    tau += -log(u) * inv_rate;
+    m1 = w.y;
+    m2 = w.z;
+  }
+  __device__ __forceinline__ void init(uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi,
+                                       double rate, double inv_rate) {
+    j = 0;
+    tau = 0.0;
+    if (rate > 0.0) {
+      draw(k0, k1, plo, phi, inv_rate);
+    } else {
+      tau = __builtin_inf();
+      m1 = m2 = 0;
+    }
+  }
+};
+
+// Step index (on a grid of n steps over [0,T]) in which time tau falls.
+__device__ __forceinline__ int32_t step_of(double tau, double n_over_T, int32_t n) {
+  const double q = tau * n_over_T;
+  return q < (double)n ? (int32_t)q : 0x7fffffff;
+}
+
+// ---------------------------------------------------------------------------
+// Chunk reduction: fp64 butterfly over the 64 lanes, lane 0 stores.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void chunk_store(double s, double q, int flag, double* partial2,
+                                            int32_t* cflag) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    q += __shfl_xor(q, off);
+  }
+  const int any_flag = __any(flag) ? 1 : 0;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    partial2[0] = s;
+    partial2[1] = q;
+    cflag[0] = any_flag;
+  }
+}
+
+// Fixed-order reduction of chunk partials into {sum, sumsq, n, flags}: thread
+// t takes chunks t, t+256, ...; then a 256-leaf LDS tree.
+__device__ __forceinline__ void finalize_block(const double* partials, const int32_t* cflags, int64_t base,
+                               int64_t nc, uint64_t n, double* st) {
+  __shared__ double ss[256];
+  __shared__ double qs[256];
+  __shared__ int fs[256];
+  const int t = threadIdx.x;
+  double s = 0.0, q = 0.0;
+  int fl = 0;
+  for (int64_t c = t; c < nc; c += 256) {
+    s += partials[2 * (base + c)];
+    q += partials[2 * (base + c) + 1];
+    fl |= cflags[base + c];
+  }
+  ss[t] = s;
+  qs[t] = q;
+  fs[t] = fl;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      ss[t] += ss[t + w];
+      qs[t] += qs[t + w];
+      fs[t] |= fs[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    st[0] = ss[0];
+    st[1] = qs[0];
+    st[2] = (double)n;
+    st[3] = (double)fs[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sources of randomness.  RNG: Philox in registers.  Fixed: host arrays.
+// ---------------------------------------------------------------------------
+struct PathCtx {
+  uint64_t path;    // global Philox path index
+  int64_t p;        // index within the job
+  int64_t P;        // paths in the job
+  uint32_t k0, k1;  // key
+};
+
+// Per-path optional stores (plain kernel only).
+struct StoreCtx {
+  float* paths;
+  float* times;
+  float* left;
+  float* jumps;
+  float* normals;
+  int64_t P;
+  int32_t rows;  // S
+};
+
+// ===========================================================================
+// Uniform grid, single level: N steps of h = fp32(T/N).  The 4K normals of
+// four steps come from K Philox blocks generated together, so every index
+// into them is a compile-time constant.
+// ===========================================================================
+template <int D, int FAM, bool STORE>
+__device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_valid, float (&x)[D],
+                                 const StoreCtx& st) {
+  constexpr int K = NK<D, FAM>::K;
+  const KModel& f = j.m;
+  const int32_t N = j.steps;
+  const float h = j.h;
+  const float sq = j.sqh;
+  const uint32_t plo = (uint32_t)c.path, phi = (uint32_t)(c.path >> 32);
+  JumpClock jc;
+  jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
+  int32_t kj = step_of(jc.tau, j.n_over_T, N);
+#pragma unroll
+  for (int i = 0; i < D; ++i) x[i] = f.x0[i];
+  if (STORE && lane_valid) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
+  }
+  for (int32_t n0 = 0; n0 < N; n0 += 4) {
+    float z[4 * K];
+    const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+      const U4 w = philox4x32_10(b0 + b, STREAM_BM, plo, phi, c.k0, c.k1);
+      box_muller(w.x, w.y, z[4 * b + 0], z[4 * b + 1]);
+      box_muller(w.z, w.w, z[4 * b + 2], z[4 * b + 3]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int32_t n = n0 + s;
+      if (n < N) {
+        float dW[D], dB[D], xo[D];
+        correlate<D, FAM>(f, &z[s * K], sq, dW, dB);
+#pragma unroll
+        for (int i = 0; i < D; ++i) xo[i] = x[i];
+        euler<D, FAM>(f, x, h, dW, dB);
+        float jsum = 0.0f;
+        if (n == kj) {
+          do {
+            const float J = jump_mark(f, jc.m1, jc.m2);
+            if (f.exact) {
+              float y[D];
+#pragma unroll
+              for (int i = 0; i < D; ++i) y[i] = x[i];
+              add_jump(f, x, y, J);
+            } else {
+              add_jump(f, x, xo, J);
+            }
+            jsum += J;
+            ++jc.j;
+            jc.draw(c.k0, c.k1, plo, phi, f.inv_rate);
+            kj = step_of(jc.tau, j.n_over_T, N);
+          } while (kj == n);
+        }
+        if (STORE && lane_valid) {
+          const int64_t r = (int64_t)(n + 1) * D;
+#pragma unroll
+          for (int i = 0; i < D; ++i) st.paths[(r + i) * st.P + c.p] = x[i];
+          if (st.jumps) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) st.jumps[(r + i) * st.P + c.p] = jsum;
+          }
+          if (st.normals) {
+            const int64_t rn = (int64_t)n * D * (FAM == FAM_MERTON ? 1 : 2);
+            if (FAM == FAM_MERTON) {
+#pragma unroll
+              for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + c.p] = dW[i];
+            } else {
+#pragma unroll
+              for (int i = 0; i < D; ++i) {
+                st.normals[(rn + 2 * i) * st.P + c.p] = dW[i];
+                st.normals[(rn + 2 * i + 1) * st.P + c.p] = dB[i];
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Row layout of fixed increments per step: D ('diag'), D + 1 ('indep'
+// common: the last row is the shared increment), 2D ('indep' full).
+template <int D, int FAM>
+struct NKF {
+  static constexpr int K = FAM == FAM_MERTON ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+};
+
+template <int D, int FAM>
+__device__ __forceinline__ void fixed_row(const float* row, int64_t P, float (&dW)[D],
+                                          float (&dB)[D]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) dW[i] = row[(int64_t)i * P];
+  if (FAM == FAM_LEVY) {
+    const float cc = row[(int64_t)D * P];
+#pragma unroll
+    for (int i = 0; i < D; ++i) dB[i] = cc;
+  } else if (FAM == FAM_DOUBLE) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) dB[i] = row[(int64_t)(D + i) * P];
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) dB[i] = 0.0f;
+  }
+}
+
+// Fixed increments on the uniform grid: dw [N][K][P] already scaled and
+// correlated, counts [N][P], marks in CSR order per path.
+template <int D, int FAM>
+__device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int64_t p, int64_t P,
+                                   float (&x)[D], int& bad) {
+  constexpr int KF = NKF<D, FAM>::K;
+  const KModel& f = j.m;
+  const int32_t N = j.steps;
+  int64_t off = in.jump_off_dev ? in.jump_off_dev[p] : 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) x[i] = f.x0[i];
+  for (int32_t n = 0; n < N; ++n) {
+    float dW[D], dB[D], xo[D];
+    fixed_row<D, FAM>(in.dw_dev + (int64_t)n * KF * P + p, P, dW, dB);
+#pragma unroll
+    for (int i = 0; i < D; ++i) xo[i] = x[i];
+    euler<D, FAM>(f, x, j.h, dW, dB);
+    const int cnt = in.counts_dev ? (int)in.counts_dev[(int64_t)n * P + p] : 0;
+    for (int k = 0; k < cnt; ++k) {
+      const bool ok = off >= 0 && off < in.n_marks;
+      const float J = ok ? in.jump_size_dev[off] : 0.0f;
+      bad |= !ok;
+      ++off;
+      if (f.exact) {
+        float y[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) y[i] = x[i];
+        add_jump(f, x, y, J);
+      } else {
+        add_jump(f, x, xo, J);
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// Uniform grid, coupled fine/coarse pair (MLMC level).  Fine steps of
+// h = fp32(T/Nf), coarse steps of hc = fp32(M T/Nf) whose increment is the
+// sum of its M fine ones; a jump in fine step n belongs to coarse step n / M.
+// Coarse jumps under the post-step rule compose into one affine map per
+// component, so any number of them in a coarse step is exact.
+// ===========================================================================
+template <int D, int FAM, class SRC>
+__device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], float (&xc)[D]) {
+  const KModel& f = j.m;
+  const int32_t M = j.M;
+  const int32_t Nc = j.steps / M;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    xf[i] = f.x0[i];
+    xc[i] = f.x0[i];
+  }
+  for (int32_t m = 0; m < Nc; ++m) {
+    float sW[D], sB[D], A[D], Bc[D];
+    float Jsum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      sW[i] = 0.0f;
+      sB[i] = 0.0f;
+      A[i] = 1.0f;
+      Bc[i] = 0.0f;
+    }
+    for (int32_t s = 0; s < M; ++s) {
+      const int32_t n = m * M + s;
+      float dW[D], dB[D], xo[D];
+      src.incr(j, n, dW, dB);
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        sW[i] += dW[i];
+        sB[i] += dB[i];
+        xo[i] = xf[i];
+      }
+      euler<D, FAM>(f, xf, j.h, dW, dB);
+      while (src.jump_pending(n)) {
+        const float J = src.take_jump(j);
+        if (f.exact) {
+          float y[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) y[i] = xf[i];
+          add_jump(f, xf, y, J);
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+            const float g = 1.0f + f.j1[i] * J;
+            A[i] = A[i] * g;
+            Bc[i] = Bc[i] * g + f.j0[i] * J;
+          }
+        } else {
+          add_jump(f, xf, xo, J);
+        }
+        Jsum += J;
+      }
+    }
+    float xco[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) xco[i] = xc[i];
+    euler<D, FAM>(f, xc, j.hc, sW, sB);
+    if (f.exact) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) xc[i] = A[i] * xc[i] + Bc[i];
+    } else if (Jsum != 0.0f) {
+      add_jump(f, xc, xco, Jsum);
+    }
+  }
+}
+
+// RNG source for the uniform coupled path.
+template <int D, int FAM>
+struct UniRngSrc {
+  NormalStream ns;
+  JumpClock jc;
+  uint32_t plo, phi, k0, k1;
+  int32_t kj;
+  __device__ __forceinline__ void init(const KJob& j, const PathCtx& c) {
+    k0 = c.k0;
+    k1 = c.k1;
+    plo = (uint32_t)c.path;
+    phi = (uint32_t)(c.path >> 32);
+    ns.init(k0, k1, c.path);
+    jc.init(k0, k1, plo, phi, j.m.rate, j.m.inv_rate);
+    kj = step_of(jc.tau, j.n_over_T, j.steps);
+  }
+  __device__ __forceinline__ void incr(const KJob& j, int32_t n, float (&dW)[D], float (&dB)[D]) {
+    constexpr int K = NK<D, FAM>::K;
+    float z[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) z[k] = ns.get((uint32_t)n * K + k);
+    correlate<D, FAM>(j.m, z, j.sqh, dW, dB);
+  }
+  __device__ __forceinline__ bool jump_pending(int32_t n) const { return kj == n; }
+  __device__ __forceinline__ float take_jump(const KJob& j) {
+    const float J = jump_mark(j.m, jc.m1, jc.m2);
+    ++jc.j;
+    jc.draw(k0, k1, plo, phi, j.m.inv_rate);
+    kj = step_of(jc.tau, j.n_over_T, j.steps);
+    return J;
+  }
+};
+
+// Fixed source for the uniform coupled path (fine-grid arrays).
+template <int D, int FAM>
+struct UniFixedSrc {
+  const sdemc_fixed_in* in;
+  int64_t p, P, off;
+  int32_t cur_n, left;
+  int bad;
+  __device__ __forceinline__ void init(const sdemc_fixed_in& fin, int64_t pp, int64_t PP) {
+    in = &fin;
+    p = pp;
+    P = PP;
+    off = fin.jump_off_dev ? fin.jump_off_dev[pp] : 0;
+    cur_n = -1;
+    left = 0;
+    bad = 0;
+  }
+  __device__ __forceinline__ void incr(const KJob&, int32_t n, float (&dW)[D], float (&dB)[D]) {
+    constexpr int KF = NKF<D, FAM>::K;
+    fixed_row<D, FAM>(in->dw_dev + (int64_t)n * KF * P + p, P, dW, dB);
+    cur_n = n;
+    left = in->counts_dev ? (int)in->counts_dev[(int64_t)n * P + p] : 0;
+  }
+  __device__ __forceinline__ bool jump_pending(int32_t n) const { return n == cur_n && left > 0; }
+  __device__ __forceinline__ float take_jump(const KJob&) {
+    --left;
+    const bool ok = off >= 0 && off < in->n_marks;
+    const float J = ok ? in->jump_size_dev[off] : 0.0f;
+    bad |= !ok;
+    ++off;
+    return J;
+  }
+};
+
+// ===========================================================================
+// Jump-adapted grid (solvers.py:164-226), fp32 clock.  Every iteration is
+// one step for every unfinished path, so the normal index (step * K + k) is
+// the same across the wavefront while its lanes are active.
+// ===========================================================================
+__device__ __forceinline__ bool isclose32(float tau, float t) {
+#pragma clang fp contract(off)
+  // torch.isclose(tau, t, atol=1e-12) with the default rtol=1e-5, in fp32
+  const float err = fabsf(tau - t);
+  const float allowed = 1e-12f + fabsf(1e-5f * t);
+  return tau == t || err <= allowed;
+}
+
+template <int D, int FAM>
+struct AdaptRngSrc {
+  NormalStream ns;
+  JumpClock jc;
+  uint32_t plo, phi, k0, k1;
+  __device__ __forceinline__ void init(const KModel& f, const PathCtx& c) {
+    k0 = c.k0;
+    k1 = c.k1;
+    plo = (uint32_t)c.path;
+    phi = (uint32_t)(c.path >> 32);
+    ns.init(k0, k1, c.path);
+    jc.init(k0, k1, plo, phi, f.rate, f.inv_rate);
+  }
+  __device__ __forceinline__ float z(uint32_t row, int k) {
+    constexpr int K = NK<D, FAM>::K;
+    return ns.get(row * K + k);
+  }
+  __device__ __forceinline__ float next_time() const { return (float)jc.tau; }
+  __device__ __forceinline__ float mark(const KModel& f) const {
+    return jump_mark(f, jc.m1, jc.m2);
+  }
+  __device__ __forceinline__ void advance(const KModel& f) {
+    ++jc.j;
+    jc.draw(k0, k1, plo, phi, f.inv_rate);
+  }
+};
+
+template <int D, int FAM>
+struct AdaptFixedSrc {
+  const sdemc_fixed_in* in;
+  int64_t p, P;
+  int32_t jn;
+  __device__ __forceinline__ void init(const sdemc_fixed_in& fin, int64_t pp, int64_t PP) {
+    in = &fin;
+    p = pp;
+    P = PP;
+    jn = 0;
+  }
+  __device__ __forceinline__ float z(uint32_t row, int k) {
+    constexpr int K = NK<D, FAM>::K;
+    if ((int32_t)row >= in->n_rows) return 0.0f;
+    return in->dw_dev[((int64_t)row * K + k) * P + p];
+  }
+  __device__ __forceinline__ float next_time() const {
+    return jn < in->max_jumps ? in->jump_time_dev[(int64_t)jn * P + p] : __builtin_inff();
+  }
+  __device__ __forceinline__ float mark(const KModel&) const {
+    return jn < in->max_jumps ? in->jump_size_dev[(int64_t)jn * P + p] : 0.0f;
+  }
+  __device__ __forceinline__ void advance(const KModel&) { ++jn; }
+};
+
+// Single level on the adapted grid.  Returns the number of steps taken;
+// `overflow` is set when max_steps was reached before T.
+template <int D, int FAM, bool STORE, class SRC>
+__device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_t p,
+                                float (&x)[D], int& overflow, const StoreCtx& st) {
+  constexpr int K = NK<D, FAM>::K;
+  const KModel& f = j.m;
+  const float T32 = (float)j.T;
+  const int32_t max_steps = j.max_steps;
+  float h = j.h;
+  float t = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; ++i) x[i] = f.x0[i];
+  float tau = lane_valid ? src.next_time() : 0.0f;
+  bool active = lane_valid && (t < T32);
+  int32_t steps = 0;
+  overflow = 0;
+  if (STORE && lane_valid) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      st.paths[(int64_t)i * st.P + p] = x[i];
+      if (st.left) st.left[(int64_t)i * st.P + p] = x[i];
+    }
+    if (st.times) st.times[p] = 0.0f;
+  }
+  while (__any(active)) {
+    if (active && steps >= max_steps) {
+      overflow = 1;
+      active = false;
+    }
+    if (active) {
+      float dt, tn;
+      {
+#pragma clang fp contract(off)
+        h = fminf(h, fmaxf(T32 - t, 0.0f));
+        dt = fminf(h, tau - t);
+      }
+      const float sq = sqrtf(dt);
+      float zz[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) zz[k] = src.z((uint32_t)steps, k);
+      float dW[D], dB[D], xo[D];
+      correlate<D, FAM>(f, zz, sq, dW, dB);
+#pragma unroll
+      for (int i = 0; i < D; ++i) xo[i] = x[i];
+      euler<D, FAM>(f, x, dt, dW, dB);
+      {
+#pragma clang fp contract(off)
+        tn = t + dt;
+      }
+      t = tn;
+      ++steps;
+      if (STORE) {
+        const int64_t r = (int64_t)steps * D;
+        if (st.left) {
+#pragma unroll
+          for (int i = 0; i < D; ++i) st.left[(r + i) * st.P + p] = x[i];
+        }
+        if (st.times) st.times[(int64_t)steps * st.P + p] = t;
+        if (st.normals) {
+          const int64_t rn = (int64_t)(steps - 1) * D * (FAM == FAM_MERTON ? 1 : 2);
+          if (FAM == FAM_MERTON) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + p] = dW[i];
+          } else {
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+              st.normals[(rn + 2 * i) * st.P + p] = dW[i];
+              st.normals[(rn + 2 * i + 1) * st.P + p] = dB[i];
+            }
+          }
+        }
+      }
+      float Jap = 0.0f;
+      if (isclose32(tau, t)) {
+        const float J = src.mark(f);
+        if (f.exact) {
+          float y[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) y[i] = x[i];
+          add_jump(f, x, y, J);
+        } else {
+          add_jump(f, x, xo, J);
+        }
+        Jap = J;
+        src.advance(f);
+        tau = src.next_time();
+      }
+      if (STORE) {
+        const int64_t r = (int64_t)steps * D;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          st.paths[(r + i) * st.P + p] = x[i];
+          if (st.jumps) st.jumps[(r + i) * st.P + p] = Jap;
+        }
+      }
+      active = t < T32;
+    }
+  }
+  if (STORE && lane_valid) {
+    // rows past this path's end repeat its final state (a finished path
+    // takes zero-length steps while others run, solvers.py:182-191)
+    for (int32_t r = steps + 1; r <= st.rows; ++r) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        st.paths[((int64_t)r * D + i) * st.P + p] = x[i];
+        if (st.left) st.left[((int64_t)r * D + i) * st.P + p] = x[i];
+        if (st.jumps) st.jumps[((int64_t)r * D + i) * st.P + p] = 0.0f;
+      }
+      if (st.times) st.times[(int64_t)r * st.P + p] = t;
+    }
+  }
+  return steps;
+}
+
+// Coupled pair on the adapted grid (solvers.py:228-307): per iteration M fine
+// sub-steps toward the pending jump time, then one coarse step driven by the
+// sum of the fine increments; a jump that falls now hits both levels, with
+// the fine coefficient taken before the last fine sub-step.
+template <int D, int FAM, class SRC>
+__device__ int32_t path_adapted_coupled(const KJob& j, SRC& src, bool lane_valid,
+                                        float (&xf)[D], float (&xc)[D], int& overflow) {
+  constexpr int K = NK<D, FAM>::K;
+  const KModel& f = j.m;
+  const int32_t M = j.M;
+  const int32_t max_steps = j.max_steps;
+  const float T32 = (float)j.T;
+  float hf = j.h;
+  float hc;
+  {
+#pragma clang fp contract(off)
+    hc = (float)M * hf;
+  }
+  float tf = 0.0f, tc = 0.0f;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    xf[i] = f.x0[i];
+    xc[i] = f.x0[i];
+  }
+  float tau = lane_valid ? src.next_time() : 0.0f;
+  bool active = lane_valid && (tf < T32);
+  int32_t it = 0;
+  overflow = 0;
+  while (__any(active)) {
+    if (active && it >= max_steps) {
+      overflow = 1;
+      active = false;
+    }
+    if (active) {
+      float sW[D], sB[D], xfo[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        sW[i] = 0.0f;
+        sB[i] = 0.0f;
+        xfo[i] = xf[i];
+      }
+      for (int32_t s = 0; s < M; ++s) {
+        float dt;
+        {
+#pragma clang fp contract(off)
+          hf = fminf(hf, fmaxf(T32 - tf, 0.0f));
+          dt = fminf(hf, tau - tf);
+        }
+        const float sq = sqrtf(dt);
+        float zz[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) zz[k] = src.z((uint32_t)(it * M + s), k);
+        float dW[D], dB[D];
+        correlate<D, FAM>(f, zz, sq, dW, dB);
+#pragma unroll
+        for (int i = 0; i < D; ++i) xfo[i] = xf[i];
+        euler<D, FAM>(f, xf, dt, dW, dB);
+        {
+#pragma clang fp contract(off)
+          tf = tf + dt;
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+            sW[i] = sW[i] + dW[i];
+            sB[i] = sB[i] + dB[i];
+          }
+        }
+      }
+      float dtc;
+      {
+#pragma clang fp contract(off)
+        hc = fminf(hc, fmaxf(T32 - tc, 0.0f));
+        dtc = fminf(hc, tau - tc);
+      }
+      float xco[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) xco[i] = xc[i];
+      euler<D, FAM>(f, xc, dtc, sW, sB);
+      {
+#pragma clang fp contract(off)
+        tc = tc + dtc;
+      }
+      if (isclose32(tau, tf)) {
+        const float J = src.mark(f);
+        if (f.exact) {
+          float y[D];
+#pragma unroll
+          for (int i = 0; i < D; ++i) y[i] = xf[i];
+          add_jump(f, xf, y, J);
+#pragma unroll
+          for (int i = 0; i < D; ++i) y[i] = xc[i];
+          add_jump(f, xc, y, J);
+        } else {
+          add_jump(f, xf, xfo, J);
+          add_jump(f, xc, xco, J);
+        }
+        src.advance(f);
+        tau = src.next_time();
+      }
+      ++it;
+      active = tf < T32;
+    }
+  }
+  return it;
+}
+
+}  // namespace sdemc

@@ -1,0 +1,75 @@
+"""The batched persistent launch: each job's statistics equal its plain launch's
+bit for bit, whatever it is packed with, whatever the tile size, and on
+every replay of the captured graph."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import sde_mc_amd as sm
+from sde_mc_amd import engine as E
+from sde_mc_amd.batch import BatchRunner, BatchTune
+from tests import jobs as J
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(n=12, seed=3):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        P = int(2 ** rng.uniform(6, 16))
+        N = int(2 ** rng.uniform(3, 8))
+        grid = 'adapted' if i % 3 == 0 else 'uniform'
+        if i % 2 == 0:
+            sde, opt = J.merton_sde(), sm.EuroCall(float(rng.uniform(0.8, 1.2)))
+        else:
+            sde, opt = J.levy_rainbow_sde(), sm.Rainbow(1.0, log=True, discount=math.exp(0.02))
+        M = 2 if (i % 4 == 1 and N % 2 == 0) else 0
+        jobs.append(J.job_for(sde, float(rng.choice([0.25, 0.5, 1.0, 2.0])), N, P, grid=grid,
+                              coarse_factor=M, option=opt, disc=0.98, seed=1000 + i))
+    return jobs
+
+
+def _plain(job):
+    return E.simulate(job, E.require_device()).cpu().numpy()
+
+
+def test_batch_equals_plain_bitwise():
+    jobs = _mix()
+    plain = np.array([_plain(j) for j in jobs])
+    r = BatchRunner(max_jobs=64)
+    for tune in (BatchTune(4, 2, 1), BatchTune(1, 4, 4), BatchTune(8, 1, 2)):
+        got = r.run(jobs, tune=tune)
+        assert np.array_equal(got, plain), tune
+
+
+def test_job_alone_equals_job_in_mix():
+    jobs = _mix(10, seed=4)
+    r = BatchRunner(max_jobs=64)
+    mixed = r.run(jobs)
+    for i, j in enumerate(jobs):
+        alone = r.run([j])
+        assert np.array_equal(alone[0], mixed[i])
+
+
+def test_graph_replays_are_consistent():
+    jobs = _mix(8, seed=5)
+    r = BatchRunner(max_jobs=64, use_graph=True)
+    first = r.run(jobs)
+    for _ in range(4):
+        torch.cuda.synchronize()
+        again = r.run(jobs)
+        assert np.array_equal(again, first)
+    r2 = BatchRunner(max_jobs=64, use_graph=False)
+    assert np.array_equal(r2.run(jobs), first)
+
+
+def test_all_class_set():
+    # 3-D and 4-D models take the full class set of the persistent kernel
+    corr = torch.tensor([[1.0, 0.2, 0.1], [0.2, 1.0, 0.3], [0.1, 0.3, 1.0]])
+    jobs = [J.job_for(J.merton_sde(dim=3, corr=corr), 1.0, 32, 3000, option=sm.Rainbow(1.0), seed=7),
+            J.job_for(J.merton_sde(), 1.0, 40, 5000, option=sm.EuroCall(1.0), seed=8)]
+    plain = np.array([_plain(j) for j in jobs])
+    assert np.array_equal(BatchRunner(max_jobs=8).run(jobs), plain)

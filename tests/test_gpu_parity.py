@@ -1,0 +1,191 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the fp64 C oracle on
+the same inputs, path by path.
+
+Tolerances (fp32 stepping against fp64):
+  * terminal states: |x32 - x64| <= 2e-5 (1 + |x64|) per path, the fp32
+    rounding of ~10^3 Euler steps plus the hardware transcendentals of
+    Box-Muller (v_log/v_sqrt/v_sin/v_cos, ~1e-6 relative per normal);
+  * adapted-grid step counts: identical (the clock is fp32 on both sides);
+  * prices from the fp64 sums: the oracle mean within 1e-5 absolute.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from sde_mc_amd import _lib as L
+from sde_mc_amd import engine as E
+from sde_mc_amd import EuroCall, Rainbow
+from tests import jobs as J
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-5
+
+
+def _dev():
+    return E.require_device()
+
+
+def _run_rng(job):
+    dev = _dev()
+    P = int(job.n_paths)
+    lv = 2 if job.coarse_factor else 1
+    d = job.model.dim
+    pay = torch.empty(P, dtype=torch.float32, device=dev)
+    term = torch.empty((lv, d, P), dtype=torch.float32, device=dev)
+    ns = torch.empty(P, dtype=torch.int32, device=dev)
+    out = L.PathOut()
+    out.payoff_dev, out.terminal_dev, out.nsteps_dev = pay.data_ptr(), term.data_ptr(), ns.data_ptr()
+    print('rng job P=%d N=%d grid=%d M=%d noise=%d' % (P, job.steps, job.grid, job.coarse_factor,
+                                                      job.model.noise), flush=True)
+    stats = E.simulate(job, dev, out=out)
+    s = stats.cpu().numpy()
+    return pay.cpu().numpy(), term.cpu().numpy(), ns.cpu().numpy(), s
+
+
+def _run_fixed(job, fx):
+    dev = _dev()
+    P = int(job.n_paths)
+    lv = 2 if job.coarse_factor else 1
+    d = job.model.dim
+    pay = torch.empty(P, dtype=torch.float32, device=dev)
+    term = torch.empty((lv, d, P), dtype=torch.float32, device=dev)
+    ns = torch.empty(P, dtype=torch.int32, device=dev)
+    out = L.PathOut()
+    out.payoff_dev, out.terminal_dev, out.nsteps_dev = pay.data_ptr(), term.data_ptr(), ns.data_ptr()
+    fin = fx.device_struct(dev)
+    print('fixed job P=%d N=%d grid=%d M=%d' % (P, job.steps, job.grid, job.coarse_factor), flush=True)
+    stats = E.simulate_fixed(job, fin, dev, out=out)
+    s = stats.cpu().numpy()
+    return pay.cpu().numpy(), term.cpu().numpy(), ns.cpu().numpy(), s
+
+
+def _close(a32, b64, tol=TOL):
+    err = np.abs(a32.astype(np.float64) - b64) / (1 + np.abs(b64))
+    return float(err.max())
+
+
+# ---- generator ------------------------------------------------------------
+def test_device_philox_known_answers():
+    dev = _dev()
+    from tests.test_oracle_paths import KAT
+    for ctr, key, want in KAT:
+        c = torch.tensor([v - (1 << 32) if v >= 1 << 31 else v for v in ctr], dtype=torch.int32,
+                         device=dev)
+        o = torch.empty(4, dtype=torch.int32, device=dev)
+        L.check(L.lib().sdemc_philox_dev(ctypes.c_void_p(c.data_ptr()), key[0], key[1],
+                                         ctypes.c_void_p(o.data_ptr()), 1, E.stream_ptr()), 'philox')
+        got = [int(v) & 0xFFFFFFFF for v in o.cpu().tolist()]
+        assert got == want
+
+
+def test_device_normals_match_oracle():
+    dev = _dev()
+    P, n = 256, 64
+    out = torch.empty((n, P), dtype=torch.float32, device=dev)
+    L.check(L.lib().sdemc_normals_dev(12345, 1000, P, n, ctypes.c_void_p(out.data_ptr()),
+                                      E.stream_ptr()), 'normals')
+    z = out.cpu().numpy().astype(np.float64)
+    ref = np.array([[oracle.normal(12345, 1000 + p, i) for p in range(P)] for i in range(n)])
+    assert np.abs(z - ref).max() < 5e-6 * (1 + np.abs(ref)).max()
+
+
+# ---- fixed-increment parity -------------------------------------------------
+def test_fixed_uniform_merton_2p16_x_500():
+    # 2^16 paths x 500 steps of host N(0, dt) increments, Poisson counts and
+    # lognormal marks: every path against the fp64 oracle
+    P, N = 1 << 16, 500
+    job = J.job_for(J.merton_sde(), 1.0, N, P, option=EuroCall(1.0), disc=math.exp(-0.02))
+    fx = J.fixed_uniform(job, seed=10)
+    pay, term, _, s = _run_fixed(job, fx)
+    opay, oterm, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term[0, 0], oterm[0, 0]) < TOL
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+    assert s[2] == P and s[3] == 0
+
+
+@pytest.mark.parametrize("grid,coupled", [('uniform', 2), ('adapted', 0), ('adapted', 2)])
+def test_fixed_merton_variants(grid, coupled):
+    P, N = 4096 + 37, 64
+    job = J.job_for(J.merton_sde(rate=4.0), 1.0, N, P, grid=grid, coarse_factor=coupled,
+                    option=EuroCall(1.0), disc=math.exp(-0.02))
+    fx = J.fixed_adapted(job, seed=11) if grid == 'adapted' else J.fixed_uniform(job, seed=11)
+    pay, term, ns, s = _run_fixed(job, fx)
+    opay, oterm, ons = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term, oterm) < TOL
+    if grid == 'adapted':
+        assert np.array_equal(ns, ons)
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+
+
+@pytest.mark.parametrize("grid", ['uniform', 'adapted'])
+def test_fixed_levy_rainbow(grid):
+    P, N = 2048 + 5, 100
+    job = J.job_for(J.levy_rainbow_sde(), 1.0, N, P, grid=grid,
+                    option=Rainbow(1.0, log=True, discount=math.exp(0.02)), disc=math.exp(-0.02))
+    fx = J.fixed_adapted(job, seed=12) if grid == 'adapted' else J.fixed_uniform(job, seed=12)
+    pay, term, ns, s = _run_fixed(job, fx)
+    opay, oterm, ons = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term, oterm) < TOL
+    if grid == 'adapted':
+        assert np.array_equal(ns, ons)
+
+
+# ---- RNG-mode parity (same Philox counters on both sides) ---------------------
+CASES = [
+    ('merton', 'uniform', 0), ('merton', 'adapted', 0), ('merton', 'uniform', 2),
+    ('merton', 'adapted', 2), ('levy', 'uniform', 0), ('levy', 'adapted', 0),
+    ('levy', 'uniform', 2), ('merton3', 'uniform', 0),
+]
+
+
+@pytest.mark.parametrize("model,grid,coupled", CASES)
+def test_rng_paths_match_oracle(model, grid, coupled):
+    P, N = 1000, 96
+    if model == 'merton':
+        sde, opt = J.merton_sde(rate=3.0), EuroCall(1.0)
+    elif model == 'merton3':
+        corr = torch.tensor([[1.0, 0.3, 0.1], [0.3, 1.0, -0.2], [0.1, -0.2, 1.0]])
+        sde, opt = J.merton_sde(rate=3.0, dim=3, corr=corr), Rainbow(1.0)
+    else:
+        sde, opt = J.levy_rainbow_sde(), Rainbow(1.0, log=True, discount=math.exp(0.02))
+    job = J.job_for(sde, 1.0, N, P, grid=grid, coarse_factor=coupled, option=opt,
+                    disc=math.exp(-0.02), seed=777, path_offset=12345)
+    pay, term, ns, s = _run_rng(job)
+    opay, oterm, ons = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    if grid == 'adapted':
+        assert np.array_equal(ns, ons)
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+    assert s[2] == P
+
+
+@pytest.mark.parametrize("P", [1, 63, 64, 65])
+def test_rng_ragged_path_counts(P):
+    job = J.job_for(J.merton_sde(), 1.0, 10, P, option=EuroCall(1.0), seed=5)
+    pay, term, _, s = _run_rng(job)
+    opay, oterm, _ = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    assert s[2] == P
+
+
+def test_empty_job_gives_zero_stats():
+    job = J.job_for(J.merton_sde(), 1.0, 10, 1, option=EuroCall(1.0))
+    job.n_paths = 0
+    s = E.simulate(job, _dev()).cpu().numpy()
+    assert list(s) == [0.0, 0.0, 0.0, 0.0]
+
+
+def test_batch_size_does_not_change_results():
+    # paths are keyed by (seed, path index): two launches of 500 equal one of 1000
+    job = J.job_for(J.merton_sde(), 1.0, 50, 1000, option=EuroCall(1.0), seed=9)
+    a, ta, _, _ = _run_rng(job)
+    j1 = J.job_for(J.merton_sde(), 1.0, 50, 500, option=EuroCall(1.0), seed=9, path_offset=0)
+    j2 = J.job_for(J.merton_sde(), 1.0, 50, 500, option=EuroCall(1.0), seed=9, path_offset=500)
+    b1, t1, _, _ = _run_rng(j1)
+    b2, t2, _, _ = _run_rng(j2)
+    assert np.array_equal(a, np.concatenate([b1, b2]))

@@ -1,0 +1,62 @@
+"""The oracle's closed forms against the reference's golden values
+(tests/test_options.py of sde_mc) and against independent computations."""
+import math
+
+import pytest
+
+from oracle import closed_form as cf
+
+
+def test_bs_call_golden():
+    # sde_mc tests/test_options.py:11-14
+    assert math.isclose(cf.bs_call(1, 1, 3, 0.02, 0.2), 0.1646004265, rel_tol=1e-5)
+
+
+def test_bs_binary_aon_golden():
+    # sde_mc tests/test_options.py:5-8
+    assert math.isclose(cf.bs_binary_aon(1, 1, 3, 0.02, 0.2), 0.63548275523, rel_tol=1e-9)
+
+
+def test_merton_call_golden():
+    # sde_mc tests/test_options.py:17-20: 0.36328189504657027
+    assert math.isclose(cf.merton_call(1, 1, 3, 0.02, 0.3, -0.05, 0.3, 2), 0.36328189504657027,
+                        rel_tol=1e-13)
+
+
+@pytest.mark.parametrize("args", [
+    (1, 1, 3, 0.02, 0.3, -0.05, 0.3, 2),
+    (1, 1, 1, 0.02, 0.2, -0.1, 0.15, 2.0),
+    (1, 0.8, 0.25, 0.02, 0.2, -0.1, 0.15, 2.0),
+    (1, 1.2, 2.0, 0.05, 0.35, 0.05, 0.4, 0.5),
+])
+def test_merton_series_matches_fourier(args):
+    # two unrelated derivations: Poisson mixture of BS prices vs Gil-Pelaez
+    # inversion of the characteristic function
+    assert abs(cf.merton_call(*args) - cf.merton_call_fourier(*args)) < 1e-9
+
+
+def test_merton_config_value():
+    # the Merton call of the 1-D workloads (S0=K=1, r=0.02, sigma=0.2, T=1,
+    # lambda=2, log-jump N(-0.1, 0.15^2))
+    assert math.isclose(cf.merton_call(1, 1, 1, 0.02, 0.2, -0.1, 0.15, 2.0),
+                        0.13205167640082938, rel_tol=1e-13)
+
+
+def test_merton_reduces_to_bs():
+    assert abs(cf.merton_call(1, 1.1, 2, 0.03, 0.25, -0.1, 0.15, 0.0) -
+               cf.bs_call(1, 1.1, 2, 0.03, 0.25)) < 1e-15
+
+
+@pytest.mark.parametrize("args", [
+    (1, 1, 1, 1, 0.02, 0.2, 0.3, 0.5),
+    (1.1, 0.9, 1.05, 2, 0.03, 0.25, 0.15, -0.3),
+    (1, 1, 0.8, 0.25, 0.02, 0.2, 0.3, 0.9),
+])
+def test_stulz_matches_quadrature(args):
+    assert abs(cf.stulz_call_max(*args) - cf.call_max_quadrature(*args)) < 1e-10
+
+
+def test_stulz_limits():
+    # rho -> 1 with equal vols: max of two identical assets is one asset
+    v = cf.call_max_quadrature(1, 1, 1, 1, 0.02, 0.2, 0.2, 0.999999)
+    assert abs(v - cf.bs_call(1, 1, 1, 0.02, 0.2)) < 2e-4

@@ -1,0 +1,238 @@
+"""Pins of the C oracle's path restatement, each against something the
+oracle's own code does not contain: known-answer vectors, closed forms,
+product formulas, hand-built event grids, the torch-fp32 restatement of the
+reference loop, statistical laws."""
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy import stats
+from scipy.integrate import quad
+
+import oracle
+from oracle import closed_form as cf
+from oracle import refcpu
+from sde_mc_amd import _lib as L
+from sde_mc_amd import EuroCall, Rainbow
+from tests import jobs as J
+
+
+# ---- generator -----------------------------------------------------------
+KAT = [  # Random123 known-answer vectors for philox4x32-10
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert oracle.philox(ctr, key) == want
+
+
+def test_normals_are_standard_normal():
+    z = np.array([oracle.normal(7, p, i) for p in range(400) for i in range(100)])
+    assert abs(z.mean()) < 4 / math.sqrt(z.size)
+    assert abs(z.var() - 1) < 4 * math.sqrt(2 / z.size)
+    assert stats.kstest(z, 'norm').pvalue > 1e-3
+    # distinct streams per path and per index
+    assert oracle.normal(7, 0, 0) != oracle.normal(7, 1, 0) != oracle.normal(8, 0, 0)
+
+
+# ---- deterministic path pins ----------------------------------------------
+def _gbm_job(P, N, grid='uniform', mu=0.05, sigma=0.3, T=1.0):
+    import sde_mc_amd as sm
+    sde = sm.Gbm(mu, sigma, torch.tensor([1.0]), 1)
+    return J.job_for(sde, T, N, P, grid=grid)
+
+
+def test_zero_vol_gives_compound_drift():
+    job = _gbm_job(8, 50, sigma=0.0)
+    fx = J.fixed_uniform(job, seed=1)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), 8)
+    assert np.allclose(term[0, 0], (1 + 0.05 / 50) ** 50, rtol=1e-14)
+
+
+def test_gbm_fixed_matches_product_formula():
+    P, N = 16, 40
+    job = _gbm_job(P, N)
+    fx = J.fixed_uniform(job, seed=2)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    dw = fx.a['dw'][:, 0, :].astype(np.float64)
+    want = np.prod(1 + 0.05 / N + 0.3 * dw, axis=0)
+    assert np.allclose(term[0, 0], want, rtol=1e-13)
+
+
+def test_single_forced_jump_uniform():
+    # one jump of mark 0.5 in step 3: x_4 = x_3 (1 + mu h + sigma dW_3 + J)
+    P, N = 4, 10
+    job = J.job_for(J.merton_sde(), 1.0, N, P)
+    fx = J.fixed_uniform(job, seed=3)
+    fx.a['counts'][:] = 0
+    fx.a['counts'][3, :] = 1
+    fx.a['off'] = np.arange(P + 1, dtype=np.int64)
+    fx.a['js'] = np.full(P, 0.5, dtype=np.float32)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    a1 = job.model.a1[0]
+    dw = fx.a['dw'][:, 0, :].astype(np.float64)
+    f = 1 + a1 / N + 0.2 * dw
+    f[3] += 0.5
+    assert np.allclose(term[0, 0], np.prod(f, axis=0), rtol=1e-13)
+
+
+def test_adapted_grid_restarts_at_jump():
+    # h = 0.25, one jump at t = 0.3: steps end at .25, .3 (jump), .55, .8, 1.0
+    P = 3
+    job = J.job_for(J.merton_sde(sigma=0.0), 1.0, 4, P, grid='adapted')
+    fx = J.fixed_adapted(job, seed=4)
+    fx.a['jt'][:] = 5.0
+    fx.a['jt'][0, :] = np.float32(0.3)
+    fx.a['js'][0, :] = 0.25
+    _, term, ns = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert (ns == 5).all()
+    a1 = job.model.a1[0]
+    f32 = np.float32
+    t1 = f32(0.25)
+    t2 = f32(0.3)                      # 0.25 + (0.3 - 0.25) is exact
+    t3 = f32(t2 + f32(0.25))
+    t4 = f32(t3 + f32(0.25))
+    dts = [t1, f32(t2 - t1), f32(0.25), f32(0.25), f32(f32(1.0) - t4)]
+    x = 1.0
+    for k, dt in enumerate(dts):
+        xo = x
+        x = x + a1 * x * float(dt)
+        if k == 1:                     # the jump lands at the end of step 2
+            x += xo * 0.25
+    assert np.allclose(term[0, 0], x, rtol=1e-7)
+
+
+def test_adapted_matches_torch_fp32_restatement():
+    # the C oracle's event grid equals the op-for-op torch-fp32 loop of the
+    # reference on the same fixed draws: same step counts, states within fp32
+    P, N = 64, 20
+    job = J.job_for(J.merton_sde(rate=6.0), 1.0, N, P, grid='adapted')
+    fx = J.fixed_adapted(job, seed=5)
+    _, term, ns = oracle.simulate_fixed(job, fx.host_struct(), P)
+    m = refcpu.merton_model(0.02, 0.2, 6.0, -0.1, 0.15)
+    paths, total = refcpu.solve_adapted(m, 1.0, N, P, int(job.max_steps) - N,
+                                        fixed=(fx.a['dw'], fx.a['jt'], fx.a['js']))
+    assert total == ns.max()
+    assert np.allclose(paths[:, total, 0].numpy(), term[0, 0], rtol=2e-5)
+
+
+def test_coupled_coarse_equals_single_on_summed_increments():
+    # the coarse path of a coupled pair is a single-level run on the coarse
+    # grid whose increments are pairwise sums and whose jumps are the fine ones
+    P, N, M = 32, 24, 2
+    sde = J.merton_sde(rate=5.0)
+    cj = J.job_for(sde, 1.0, N, P, coarse_factor=M)
+    fx = J.fixed_uniform(cj, seed=6)
+    _, tc, _ = oracle.simulate_fixed(cj, fx.host_struct(), P)
+    sj = J.job_for(sde, 1.0, N // M, P)
+    dw = fx.a['dw'].astype(np.float64)
+    sfx = J.FixedInputs(dict(dw=(dw[0::2] + dw[1::2]).astype(np.float32),
+                             counts=(fx.a['counts'][0::2] + fx.a['counts'][1::2]).astype(np.uint8),
+                             off=fx.a['off'], js=fx.a['js']), N // M)
+    _, ts, _ = oracle.simulate_fixed(sj, sfx.host_struct(), P)
+    # the summed fp32 increments differ from the fp64 sums by one rounding
+    assert np.allclose(tc[1, 0], ts[0, 0], rtol=1e-6)
+    _, tf, _ = oracle.simulate_fixed(J.job_for(sde, 1.0, N, P), fx.host_struct(), P)
+    assert np.array_equal(tc[0, 0], tf[0, 0])
+
+
+# ---- Levy inverse CDF ----------------------------------------------------
+def _levy_cdf(z, cm, cp, mu, alpha, eps, lda):
+    im = 0.0 if math.isinf(mu) else 1.0 / mu
+
+    def dens(x):
+        ax = abs(x)
+        c = cm if x < 0 else cp
+        if ax < eps:
+            return 0.0
+        if ax <= 1:
+            return c * ax ** (-1 - alpha) / lda
+        return c * math.exp(-mu * (ax - 1)) / lda if im > 0 else 0.0
+    lo = -1.0 if im == 0 else -60.0
+    pts = [p for p in (-1.0, -eps, eps, 1.0) if lo < p < z]
+    return quad(dens, lo, z, points=pts or None, limit=200, epsabs=1e-13)[0]
+
+
+@pytest.mark.parametrize("mu", [2.0, math.inf])
+def test_levy_icdf_inverts_density(mu):
+    import sde_mc_amd as sm
+    icdf = sm.InverseCdf(1.0, 1.0, mu, 0.8, 0.05)
+    sde = sm.LevySde(sm.ExpExampleLevy(1.0, 1.0, 0.8, mu, 0.02, 0.2, 1.0, 0.05, 1),
+                     torch.tensor([1.0]))
+    job = J.job_for(sde, 1.0, 10, 1)
+    zs = [-0.9, -0.3, -0.06, 0.06, 0.3, 0.9] + ([-1.5, 1.7] if not math.isinf(mu) else [])
+    for z in zs:
+        y = _levy_cdf(z, 1.0, 1.0, mu, 0.8, 0.05, icdf.lda)
+        assert abs(oracle.levy_icdf(job, y) - z) < 2e-5 * max(1, abs(z)), (z, y)
+
+
+# ---- statistical pins (whole RNG-mode path) ------------------------------
+def test_oracle_rng_merton_price_uniform():
+    n = 1 << 16
+    job = J.job_for(J.merton_sde(), 1.0, 50, n, option=EuroCall(1.0), disc=math.exp(-0.02))
+    pay, _, _ = oracle.simulate(job, 0, n)
+    se = pay.std() / math.sqrt(n)
+    assert abs(pay.mean() - J.MERTON_PRICE) < 4 * se + 2e-4
+
+
+def test_oracle_rng_merton_price_adapted():
+    n = 1 << 16
+    job = J.job_for(J.merton_sde(), 1.0, 50, n, grid='adapted', option=EuroCall(1.0),
+                    disc=math.exp(-0.02))
+    pay, _, ns = oracle.simulate(job, 0, n)
+    se = pay.std() / math.sqrt(n)
+    assert abs(pay.mean() - J.MERTON_PRICE) < 4 * se + 2e-4
+    # a path with k jumps before T takes between N + 1 and N + k grid steps
+    # (k + 1 segments, each rounded up to whole steps), about N + (k + 1) / 2
+    assert ns.min() >= 50 and 50 + 1.2 < ns.mean() < 50 + 2.0
+
+
+def test_oracle_rng_levy_martingale():
+    # exp-Levy log price with compensated drift: E[exp(X_T)] = 1 per asset
+    n = 1 << 14
+    sde = J.levy_rainbow_sde()
+    job = J.job_for(sde, 1.0, 50, n)
+    _, term, _ = oracle.simulate(job, 0, n)
+    for i in range(2):
+        e = np.exp(term[0, i])
+        assert abs(e.mean() - 1.0) < 4 * e.std() / math.sqrt(n) + 1e-3
+
+
+def test_oracle_rng_levy_jump_free_matches_stulz():
+    # rate 0 and f = 0: the log-price Euler scheme is exact, so the call on the
+    # max equals the two-asset closed form
+    n = 1 << 15
+    sde = J.levy_rainbow_sde(f=(0.0, 0.0))
+    model = sde.kernel_model()
+    model.rate = 0.0
+    from sde_mc_amd import engine as E
+    job = E.make_job(model, 1.0, 20, L.GRID_UNIFORM, n, 99, 0)
+    E.fill_payoff(job, Rainbow(1.0, log=True, discount=math.exp(0.02)), math.exp(-0.02))
+    pay, _, _ = oracle.simulate(job, 0, n)
+    want = cf.stulz_call_max(1, 1, 1, 1, 0.02, 0.2, 0.3, 0.5)
+    assert abs(pay.mean() - want) < 4 * pay.std() / math.sqrt(n)
+
+
+def test_oracle_mlmc_telescopes():
+    # E[P_fine - P_coarse] from the coupled pair equals the difference of the
+    # two single-level prices
+    n = 1 << 15
+    sde = J.merton_sde()
+    opt, disc = EuroCall(1.0), math.exp(-0.02)
+    c = J.job_for(sde, 1.0, 16, n, coarse_factor=2, option=opt, disc=disc)
+    d, _, _ = oracle.simulate(c, 0, n)
+    f = J.job_for(sde, 1.0, 16, n, option=opt, disc=disc, seed=11)
+    g = J.job_for(sde, 1.0, 8, n, option=opt, disc=disc, seed=12)
+    pf, _, _ = oracle.simulate(f, 0, n)
+    pg, _, _ = oracle.simulate(g, 0, n)
+    diff = pf.mean() - pg.mean()
+    se = math.sqrt(pf.var() / n + pg.var() / n)
+    assert abs(d.mean() - diff) < 4 * se
+    # the coupled correction has far smaller variance than either level
+    assert d.var() < 0.05 * pf.var()
