@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "path-steps/s batched vs plain at equal CI; p50/p99 pricing latency"
-PEAK_VALU_OPS = 256 * 64 * 2.4e9          # lane-ops/s: 256 CUs x 4 SIMD16 x 2.4 GHz
+PEAK_VALU_OPS = 78.6e12                   # fp32 vector FMA lane-ops/s (157.3 TFLOP/s / 2)
 PEAK_FP32 = 157.3e12
 
 
@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X per process and "
@@ -89,7 +90,7 @@ def main():
     total_ps = sum(r.path_steps for r in reqs)
     ops = sum(r.path_steps * W.valu_ops_per_path_step(r.model, r.steps, r.maturity) for r in reqs)
     chunks = sum(chunks_of(r.paths) for r in reqs)
-    runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks, max_tiles=max(chunks, 1024))
+    runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks)
 
     for _ in range(args.warmup):
         ref = runner.run(jobs)
@@ -119,6 +120,24 @@ def main():
         plain = total_ps / dtp
         assert np.array_equal(np.array(pst), st), "plain and batched results differ"
 
+    latency = None
+    if not args.no_serve:
+        from sde_mc_amd.serve import PricingServer, latency_summary, serve_plain
+        latency = {}
+        for rate in (50, 200, 1000):
+            arr = W.request_mix(args.requests, seed=args.seed + rate, rate=rate)
+            srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20)
+            res = srv.serve(arr)
+            served = latency_summary(res)
+            served["rounds"] = srv.rounds
+            served["peak_partial_bytes"] = srv.peak_used * 20
+            arr2 = W.request_mix(args.requests, seed=args.seed + rate, rate=rate)
+            plain_res = serve_plain(arr2)
+            same = all(np.array_equal(np.array(res[i].stats[:3]), np.array(plain_res[i].stats[:3]))
+                       for i in res)
+            latency[str(rate)] = {"served": served, "plain": latency_summary(plain_res),
+                                  "results_identical": bool(same)}
+
     cpu = None if args.no_cpu else cpu_baseline(reqs)
 
     achieved = ops / (kern_ms * 1e-3)
@@ -142,6 +161,7 @@ def main():
                    "launch": "batched persistent kernel, graph-replayed"},
         "plain_value": plain,
         "speedup_vs_plain": (batched / plain) if plain else None,
+        "latency": latency,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
                      "unit": "Top/s (VALU lane-ops, algorithmic)",
                      "frac": achieved / PEAK_VALU_OPS, "traffic": None,

@@ -223,19 +223,25 @@ int sdemc_reduce_finalize(const double* partials_dev, const int32_t* chunk_flags
 /* ---- batched persistent launch ------------------------------------------
  * A job table of max_jobs slots lives on the device in the library's own
  * prepared layout (sdemc_sizeof_prepared_job() bytes per slot), filled by
- * the caller with bytes from sdemc_prepare_jobs.  A launch runs a tile list:
- * tile t covers chunks [first, first + count) of slot `job`.  Persistent
- * wavefronts take tiles from a device counter until the list is exhausted,
- * so jobs of any size and any mix of models, grids and MLMC levels share
- * one launch.  Partials of slot j live at partials[chunk_base_j + c].
- * Tiles naming an unprepared slot or chunks outside a slot are skipped.
+ * the caller with bytes from sdemc_prepare_jobs.  A launch runs a list of
+ * segments: segment s covers chunks [first, first + count) of slot `job`,
+ * cut into tiles of `per_tile` chunks; segment s owns tiles
+ * [tile_begin, tile_begin + ceil(count / per_tile)), the segments' tile
+ * ranges are consecutive from 0, and the list order is the order in which
+ * tiles are handed out.  Persistent wavefronts take tile indices from a
+ * device counter until all tiles are taken, so jobs of any size and any mix
+ * of models, grids and MLMC levels share one launch.  Partials of slot j
+ * live at partials[chunk_base_j + c].  Segments naming an unprepared slot or
+ * chunks outside a slot are skipped.
  */
-typedef struct sdemc_tile {
-  int32_t job;   /* slot index                        */
-  int32_t first; /* first chunk                       */
-  int32_t count; /* chunks in this tile (>= 1)        */
-  int32_t cost;  /* scheduling weight (host use only) */
-} sdemc_tile;
+typedef struct sdemc_segment {
+  int32_t job;        /* slot index                                     */
+  int32_t first;      /* first chunk                                    */
+  int32_t count;      /* chunks in the segment                          */
+  int32_t per_tile;   /* chunks per tile (>= 1)                         */
+  int32_t tile_begin; /* index of the segment's first tile              */
+  int32_t reserved;
+} sdemc_segment;
 
 /* Bytes of one prepared job slot. */
 size_t sdemc_sizeof_prepared_job(void);
@@ -250,8 +256,9 @@ int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_ba
 
 typedef struct sdemc_batch_dev {
   const void* jobs_dev;          /* [max_jobs] prepared slots                   */
-  const sdemc_tile* tiles_dev;   /* [max_tiles]                                 */
-  const int32_t* n_tiles_dev;    /* [1] tiles in this launch (read on device)   */
+  const sdemc_segment* segs_dev; /* [max_segments]                              */
+  const int32_t* seg_hdr_dev;    /* [2] {segments, tiles} of this launch (read on
+                                    the device)                                 */
   const int32_t* finalize_dev;   /* [max_jobs] nonzero: finalise slot this launch */
   double* partials_dev;          /* [partial_chunks][2]                         */
   int32_t* chunk_flags_dev;      /* [partial_chunks] overflow flags             */
@@ -259,7 +266,7 @@ typedef struct sdemc_batch_dev {
   int32_t* counter_dev;          /* [1] tile counter, zeroed by the launch      */
   int64_t partial_chunks;        /* capacity of partials / chunk_flags          */
   int32_t max_jobs;
-  int32_t max_tiles;
+  int32_t max_segments;
   int32_t class_set;             /* from sdemc_prepare_jobs                     */
   int32_t reserved;
 } sdemc_batch_dev;
@@ -276,8 +283,9 @@ typedef struct sdemc_batch_tune {
 int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream);
 
 /* The same three launches captured once into a HIP graph and replayed by
- * sdemc_batch_plan_launch; the graph reads n_tiles and the tables from the
- * device, so one plan serves every launch that uses the same buffers. */
+ * sdemc_batch_plan_launch; the graph reads the segment header and the
+ * tables from the device, so one plan serves every launch that uses the
+ * same buffers. */
 typedef struct sdemc_batch_plan sdemc_batch_plan;
 int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tune,
                             sdemc_batch_plan** plan);

@@ -64,19 +64,18 @@ class FixedIn(ctypes.Structure):
     ]
 
 
-class Tile(ctypes.Structure):
-    _fields_ = [("job", ctypes.c_int32), ("first", ctypes.c_int32),
-                ("count", ctypes.c_int32), ("cost", ctypes.c_int32)]
+SEGMENT_INTS = 6  # sdemc_segment: job, first, count, per_tile, tile_begin, reserved
 
 
 class BatchDev(ctypes.Structure):
     _fields_ = [
-        ("jobs_dev", ctypes.c_void_p), ("tiles_dev", ctypes.c_void_p),
-        ("n_tiles_dev", ctypes.c_void_p), ("finalize_dev", ctypes.c_void_p),
+        ("jobs_dev", ctypes.c_void_p), ("segs_dev", ctypes.c_void_p),
+        ("seg_hdr_dev", ctypes.c_void_p), ("finalize_dev", ctypes.c_void_p),
         ("partials_dev", ctypes.c_void_p), ("chunk_flags_dev", ctypes.c_void_p),
         ("stats_dev", ctypes.c_void_p), ("counter_dev", ctypes.c_void_p),
         ("partial_chunks", ctypes.c_int64), ("max_jobs", ctypes.c_int32),
-        ("max_tiles", ctypes.c_int32), ("class_set", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("max_segments", ctypes.c_int32), ("class_set", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

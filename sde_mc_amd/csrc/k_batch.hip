@@ -1,8 +1,9 @@
-// k_batch.hip -- the batched persistent launch: one kernel runs a tile list
-// spanning any number of jobs (different models, grids, path counts, step
-// counts and MLMC levels).  Each wavefront repeatedly takes the next tile
-// from a device counter, so long and short tiles balance across the 256 CUs
-// without host involvement; the host orders tiles longest-first.
+// k_batch.hip -- the batched persistent launch: one kernel runs a segment
+// list spanning any number of jobs (different models, grids, path counts,
+// step counts and MLMC levels).  Each wavefront repeatedly takes the next
+// tile index from a device counter and finds its segment by binary search,
+// so long and short tiles balance across the 256 CUs without host
+// involvement; the host orders segments costliest-first.
 #include <string.h>
 
 #include "sdemc_kernels.h"
@@ -110,30 +111,45 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
 
 template <int SET>
 __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
-                                               const sdemc_tile* __restrict__ tiles,
-                                               const int32_t* __restrict__ n_tiles_dev,
+                                               const sdemc_segment* __restrict__ segs,
+                                               const int32_t* __restrict__ hdr,
                                                double* __restrict__ partials,
                                                int32_t* __restrict__ cflags,
                                                int32_t* __restrict__ counter,
                                                int64_t partial_chunks, int32_t max_jobs,
-                                               int32_t max_tiles) {
+                                               int32_t max_segments) {
   const int lane = threadIdx.x & (WAVE - 1);
-  int32_t n_tiles = *n_tiles_dev;
-  n_tiles = n_tiles < max_tiles ? n_tiles : max_tiles;
+  int32_t nseg = hdr[0];
+  nseg = nseg < max_segments ? nseg : max_segments;
+  const int32_t ntiles = hdr[1];
+  if (nseg <= 0) return;
+  int32_t seg = 0;
   for (;;) {
     int32_t t = 0;
     if (lane == 0) t = atomicAdd(counter, 1);
     t = __shfl(t, 0);
     t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= n_tiles) break;
-    const sdemc_tile tile = tiles[t];
-    if (tile.job < 0 || tile.job >= max_jobs) continue;
-    const KJob& job = jobs[tile.job];
+    if (t >= ntiles) break;
+    // segment owning tile t: the last one whose tile_begin <= t; a wave's
+    // tile indices only grow, so search [seg, nseg)
+    int32_t lo = seg, hi = nseg - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (segs[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    seg = lo;
+    const sdemc_segment sg = segs[seg];
+    if (sg.job < 0 || sg.job >= max_jobs || sg.per_tile < 1) continue;
+    const KJob& job = jobs[sg.job];
     if (job.magic != KJOB_MAGIC) continue;
     const int64_t nc = (int64_t)((job.n_paths + WAVE - 1) / WAVE);
-    int64_t c0 = tile.first < 0 ? 0 : tile.first;
-    int64_t c1 = (int64_t)tile.first + (tile.count > 0 ? tile.count : 0);
+    const int64_t seg_end = (int64_t)sg.first + (sg.count > 0 ? sg.count : 0);
+    int64_t c0 = (int64_t)sg.first + (int64_t)(t - sg.tile_begin) * sg.per_tile;
+    int64_t c1 = c0 + sg.per_tile;
+    if (c1 > seg_end) c1 = seg_end;
     if (c1 > nc) c1 = nc;
+    if (c0 < 0) c0 = 0;
     for (int64_t chunk = c0; chunk < c1; ++chunk) {
       const int64_t slot = job.chunk_base + chunk;
       if (slot < 0 || slot >= partial_chunks) break;
@@ -164,10 +180,10 @@ __global__ __launch_bounds__(256) void k_finalize_slots(const KJob* __restrict__
 
 static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {
   if (!b || !tune) return SDEMC_ERR_ARG;
-  if (!b->jobs_dev || !b->tiles_dev || !b->n_tiles_dev || !b->finalize_dev || !b->partials_dev ||
+  if (!b->jobs_dev || !b->segs_dev || !b->seg_hdr_dev || !b->finalize_dev || !b->partials_dev ||
       !b->chunk_flags_dev || !b->stats_dev || !b->counter_dev)
     return SDEMC_ERR_ARG;
-  if (b->max_jobs < 1 || b->max_tiles < 1 || b->partial_chunks < 0) return SDEMC_ERR_ARG;
+  if (b->max_jobs < 1 || b->max_segments < 1 || b->partial_chunks < 0) return SDEMC_ERR_ARG;
   if (b->class_set != 0 && b->class_set != 1) return SDEMC_ERR_ARG;
   const int w = tune->waves_per_block;
   if (w != 1 && w != 2 && w != 4 && w != 8) return SDEMC_ERR_ARG;
@@ -194,13 +210,13 @@ static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
   const unsigned grid = (unsigned)(cu_count_cached() * tune.blocks_per_cu);
   const KJob* jobs = (const KJob*)b.jobs_dev;
   if (b.class_set == 0) {
-    hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.tiles_dev,
-                       b.n_tiles_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
-                       b.partial_chunks, b.max_jobs, b.max_tiles);
+    hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.segs_dev,
+                       b.seg_hdr_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
+                       b.partial_chunks, b.max_jobs, b.max_segments);
   } else {
-    hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.tiles_dev,
-                       b.n_tiles_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
-                       b.partial_chunks, b.max_jobs, b.max_tiles);
+    hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.segs_dev,
+                       b.seg_hdr_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
+                       b.partial_chunks, b.max_jobs, b.max_segments);
   }
   hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s, jobs,
                      b.finalize_dev, b.partials_dev, b.chunk_flags_dev, b.partial_chunks,

@@ -1,0 +1,293 @@
+"""Continuous-batching pricing server on one GPU.
+
+Requests (model, strike, maturity, steps, paths, seed) arrive over time.  A
+request is admitted when a job slot is free and its chunk partials fit in
+the HBM budget; its paths are then priced in rounds.  Each round is one
+batched persistent launch (sde_mc_amd.batch) over a fair share of every
+admitted request's remaining 64-path chunks, so requests that arrive while
+others are on the GPU join the next round and their work shares the device
+with everything else in flight.  Two rounds are kept in flight so the host
+plans round r+1 while round r runs.  A request's partial sums stay on the
+device until its last chunk is done; it is then finalised in-kernel and its
+(sum, sum of squares, n) is read back.
+
+Results do not depend on the mix: each request's paths are keyed by its own
+seed and path index, and its reduction order is fixed (see batch.py), so a
+request priced alone and in any mix gives the same bits.
+"""
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import engine as E
+from .batch import BatchRunner, BatchTune, chunks_of, job_cost, plan_segments
+from .workloads import request_job
+
+PARTIAL_BYTES = 2 * 8 + 4  # one chunk: fp64 (sum, sum of squares) + int32 flag
+
+
+class ChunkPool:
+    """First-fit allocator of contiguous chunk-partial ranges (the HBM budget)."""
+
+    def __init__(self, capacity):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)]  # sorted (start, length)
+        self.used = 0
+
+    def alloc(self, n):
+        for i, (s, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + n, ln - n)
+                self.used += n
+                return s
+        return None
+
+    def release(self, start, n):
+        self.used -= n
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((s, ln))
+        self.free = merged
+
+
+@dataclass
+class Result:
+    rid: int
+    mean: float
+    se: float
+    n: int
+    arrival: float
+    admitted: float
+    finished: float
+    path_steps: int
+    stats: tuple = field(default=None, repr=False)
+    error: str = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class _Active:
+    __slots__ = ('req', 'job', 'slot', 'base', 'nc', 'next', 'cost', 'admitted')
+
+
+class PricingServer:
+    """Admission by slots and HBM budget, fair-share rounds of batched launches."""
+
+    def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=2.0, tune=None,
+                 grid=L.GRID_UNIFORM, job_builder=None):
+        self.pool = ChunkPool(hbm_budget_bytes // PARTIAL_BYTES)
+        self.runner = BatchRunner(max_jobs=max_slots, max_segments=2 * max_slots,
+                                  partial_chunks=self.pool.capacity, tune=tune or BatchTune(),
+                                  n_buffers=2)
+        self.max_slots = max_slots
+        self.free_slots = list(range(max_slots))[::-1]
+        self.queue = deque()
+        self.active = {}
+        self.round_s = round_ms * 1e-3
+        self.cost_rate = 1.2e12     # cost units (VALU slots) per second, refined per round
+        self.grid = grid
+        self.job_builder = job_builder or (lambda r: request_job(r, grid))
+        self.in_flight = deque()
+        self.results = {}
+        self.peak_used = 0
+        self.rounds = 0
+        self.t0 = None
+
+    # -- admission -------------------------------------------------------------
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _now(self):
+        return time.perf_counter() - self.t0
+
+    def _admit(self):
+        staged, cls = [], 0
+        while self.queue and self.free_slots:
+            req = self.queue[0]
+            nc = chunks_of(req.paths)
+            if nc > self.pool.capacity:
+                self.queue.popleft()
+                now = self._now()
+                self.results[req.rid] = Result(req.rid, math.nan, math.nan, 0, req.arrival, now,
+                                               now, 0, error='exceeds the HBM budget')
+                continue
+            base = self.pool.alloc(nc)
+            if base is None:
+                break  # wait for memory to free up (FIFO: no overtaking)
+            self.queue.popleft()
+            slot = self.free_slots.pop()
+            a = _Active()
+            a.req, a.slot, a.base, a.nc, a.next = req, slot, base, nc, 0
+            a.job = self.job_builder(req)
+            a.cost = job_cost(a.job)
+            a.admitted = self._now()
+            self.active[slot] = a
+            staged.append(a)
+        self.peak_used = max(self.peak_used, self.pool.used)
+        if staged:
+            cls = self.runner.prepare([a.job for a in staged], [a.base for a in staged],
+                                      [a.slot for a in staged])
+            self.runner.upload_slots([a.slot for a in staged])
+        return cls
+
+    # -- rounds ------------------------------------------------------------------
+    def _plan(self):
+        """Fair share of the round's cost budget over the admitted requests with
+        chunks left; a request finishing its chunks this round is finalised."""
+        todo = [a for a in self.active.values() if a.next < a.nc]
+        if not todo:
+            return None
+        todo.sort(key=lambda a: a.admitted)
+        budget = self.cost_rate * self.round_s
+        take = {a.slot: 0 for a in todo}
+        left = budget
+        for _ in range(3):
+            live = [a for a in todo if a.next + take[a.slot] < a.nc]
+            if not live or left <= 0:
+                break
+            share = left / len(live)
+            for a in live:
+                k = min(a.nc - a.next - take[a.slot], max(1, int(share / a.cost)))
+                take[a.slot] += k
+                left -= k * a.cost
+        slots, firsts, counts, costs, final = [], [], [], [], []
+        for a in todo:
+            k = take[a.slot]
+            if k <= 0:
+                continue
+            slots.append(a.slot)
+            firsts.append(a.next)
+            counts.append(k)
+            costs.append(a.cost)
+            a.next += k
+            if a.next >= a.nc:
+                final.append(a.slot)
+        return slots, firsts, counts, costs, final
+
+    def _launch(self, plan):
+        slots, firsts, counts, costs, final = plan
+        buf = next(b for b in self.runner.bufs if b not in [f[0] for f in self.in_flight])
+        segs, ntiles = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks)
+        self.runner.stage_round(buf, segs, ntiles, final)
+        cls = 0 if all(self._class0(self.active[s].job) for s in slots) else 1
+        start = torch.cuda.Event(enable_timing=True)
+        start.record()
+        self.runner.launch(buf, cls)
+        self.runner.read_stats(buf, final)
+        cost = float(np.dot(counts, costs))
+        self.in_flight.append((buf, final, start, cost))
+        self.rounds += 1
+
+    @staticmethod
+    def _class0(job):
+        m = job.model
+        return (m.noise == L.NOISE_DIAG and m.dim == 1) or (m.noise == L.NOISE_INDEP_COMMON
+                                                            and m.dim == 2)
+
+    def _complete(self, block):
+        """Retire finished rounds (the oldest first); block on it if asked."""
+        while self.in_flight:
+            buf, final, start, cost = self.in_flight[0]
+            if not block and not buf.done.query():
+                return
+            buf.done.synchronize()
+            block = False
+            self.in_flight.popleft()
+            ms = start.elapsed_time(buf.done)
+            if ms > 0.05:  # refine the throughput estimate that sizes rounds
+                self.cost_rate = 0.7 * self.cost_rate + 0.3 * (cost / (ms * 1e-3))
+            now = self._now()
+            st = buf.h_stats.numpy()
+            for s in final:
+                a = self.active.pop(s)
+                s4 = st[4 * s:4 * s + 4].copy()
+                err = None
+                try:
+                    E.check_flags(s4[3], 'request %d' % a.req.rid)
+                except L.SdemcError as e:
+                    err = str(e)
+                mean, se = E.mean_and_se(s4[0], s4[1], int(s4[2]))
+                self.results[a.req.rid] = Result(a.req.rid, mean, se, int(s4[2]), a.req.arrival,
+                                                 a.admitted, now, a.req.path_steps,
+                                                 tuple(s4), err)
+                self.pool.release(a.base, a.nc)
+                self.free_slots.append(s)
+
+    # -- driver ------------------------------------------------------------------
+    def serve(self, requests, realtime=True):
+        """Serve `requests` (with .arrival in seconds from start when realtime)
+        to completion; returns {rid: Result}."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        self.t0 = time.perf_counter()
+        if not realtime:
+            for r in pending:
+                r.arrival = 0.0
+        n_total = len(pending)
+        while len(self.results) < n_total:
+            now = self._now()
+            while pending and (not realtime or pending[0].arrival <= now):
+                self.submit(pending.popleft())
+            self._admit()
+            self._complete(block=False)
+            if len(self.in_flight) < len(self.runner.bufs):
+                plan = self._plan()
+                if plan is not None:
+                    self._launch(plan)
+                    continue
+            if self.in_flight:
+                self._complete(block=len(self.in_flight) == len(self.runner.bufs) or not pending)
+            elif pending:
+                wait = pending[0].arrival - self._now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.002))
+        return self.results
+
+
+def serve_plain(requests, realtime=True, grid=L.GRID_UNIFORM):
+    """The plain per-job path under the same arrivals: one launch per request in
+    arrival order, each read back before the next (mc_simple's pattern)."""
+    dev = E.require_device()
+    pending = deque(sorted(requests, key=lambda r: r.arrival))
+    t0 = time.perf_counter()
+    out = {}
+    while pending:
+        now = time.perf_counter() - t0
+        r = pending[0]
+        if realtime and r.arrival > now:
+            time.sleep(min(r.arrival - now, 0.002))
+            continue
+        pending.popleft()
+        start = time.perf_counter() - t0
+        job = request_job(r, grid)
+        s, q, n = E.stats_to_host(E.simulate(job, dev))
+        mean, se = E.mean_and_se(s, q, n)
+        fin = time.perf_counter() - t0
+        out[r.rid] = Result(r.rid, mean, se, n, r.arrival if realtime else 0.0, start, fin,
+                            r.path_steps, (s, q, n, 0.0))
+    return out
+
+
+def latency_summary(results, span=None):
+    lat = np.array([r.latency for r in results.values() if r.error is None])
+    ps = sum(r.path_steps for r in results.values() if r.error is None)
+    t_end = max(r.finished for r in results.values())
+    t_beg = min(r.arrival for r in results.values())
+    span = span or (t_end - t_beg)
+    return {"requests": len(results), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+            "throughput_path_steps_s": ps / span, "span_s": span}

@@ -1,0 +1,96 @@
+"""Host-side logic of the batched launch and the server (no device)."""
+import numpy as np
+import pytest
+
+from sde_mc_amd import _lib as L
+from sde_mc_amd.batch import chunks_of, job_cost, plan_segments
+from sde_mc_amd.serve import ChunkPool
+from sde_mc_amd import workloads as W
+from tests import jobs as J
+
+
+def test_chunks_of():
+    assert [chunks_of(n) for n in (1, 63, 64, 65, 1 << 22)] == [1, 1, 1, 2, 1 << 16]
+
+
+def test_segments_cover_every_chunk_once_costliest_first():
+    rng = np.random.default_rng(0)
+    n = 50
+    counts = rng.integers(0, 300, n)
+    firsts = rng.integers(0, 1000, n)
+    costs = rng.uniform(1, 100, n)
+    for per in (1, 2, 4, 7):
+        segs, total = plan_segments(np.arange(n), firsts, counts, costs, per)
+        assert len(segs) == int((counts > 0).sum())
+        # consecutive tile ranges from 0
+        nt = (segs[:, 2] + segs[:, 3] - 1) // segs[:, 3]
+        assert segs[0, 4] == 0 and np.array_equal(segs[1:, 4], np.cumsum(nt)[:-1])
+        assert total == nt.sum()
+        # expand tiles -> chunks: exactly the requested ranges
+        seen = {}
+        for s in segs:
+            for t in range((s[2] + s[3] - 1) // s[3]):
+                c0 = s[1] + t * s[3]
+                for c in range(c0, min(c0 + s[3], s[1] + s[2])):
+                    seen[(s[0], c)] = seen.get((s[0], c), 0) + 1
+        want = {(i, firsts[i] + c): 1 for i in range(n) for c in range(counts[i])}
+        assert seen == want
+        # costliest first
+        order_costs = costs[segs[:, 0]]
+        assert np.all(np.diff(order_costs) <= 0)
+
+
+def test_job_cost_orders_levy_above_merton():
+    m = W.request_job(W.Request(0, 'merton', 1.0, 1.0, 200, 1 << 16, 1))
+    l = W.request_job(W.Request(1, 'levy', 1.0, 1.0, 200, 1 << 16, 2))
+    assert job_cost(l) > 2 * job_cost(m)
+    c = J.job_for(J.merton_sde(), 1.0, 200, 1000, coarse_factor=2)
+    assert job_cost(c) > job_cost(J.job_for(J.merton_sde(), 1.0, 200, 1000))
+
+
+def test_chunk_pool_never_exceeds_capacity():
+    rng = np.random.default_rng(1)
+    pool = ChunkPool(1000)
+    live = []
+    for _ in range(2000):
+        if live and rng.random() < 0.45:
+            s, n = live.pop(rng.integers(len(live)))
+            pool.release(s, n)
+        else:
+            n = int(rng.integers(1, 200))
+            s = pool.alloc(n)
+            if s is not None:
+                assert 0 <= s and s + n <= 1000
+                for (s2, n2) in live:
+                    assert s + n <= s2 or s2 + n2 <= s  # no overlap
+                live.append((s, n))
+        assert pool.used == sum(n for _, n in live) <= pool.capacity
+    for s, n in live:
+        pool.release(s, n)
+    assert pool.free == [(0, 1000)] and pool.used == 0
+
+
+def test_request_mix_distribution():
+    reqs = W.request_mix(2000, seed=3, rate=200)
+    paths = np.array([r.paths for r in reqs])
+    steps = np.array([r.steps for r in reqs])
+    assert paths.min() >= 2 ** 14 and paths.max() <= 2 ** 22
+    assert steps.min() >= 50 and steps.max() <= 800
+    assert set(r.maturity for r in reqs) == {0.25, 0.5, 1.0, 2.0}
+    assert 0.4 < np.mean([r.model == 'merton' for r in reqs]) < 0.6
+    gaps = np.diff([r.arrival for r in reqs])
+    assert abs(gaps.mean() - 1 / 200) < 0.1 / 200 * 3
+    # log-uniform: the median path count is the geometric midpoint 2^18
+    assert 2 ** 17.5 < np.median(paths) < 2 ** 18.5
+
+
+def test_request_jobs_validate_on_host():
+    import ctypes
+    lib = L.load()
+    for r in W.request_mix(20, seed=4):
+        job = W.request_job(r)
+        buf = ctypes.create_string_buffer(lib.sdemc_sizeof_prepared_job())
+        base = (ctypes.c_int64 * 1)(0)
+        cs = ctypes.c_int32()
+        assert lib.sdemc_prepare_jobs(ctypes.byref(job), 1, base, buf, ctypes.byref(cs)) == L.OK
+        assert cs.value == 0
