@@ -90,7 +90,13 @@ def main():
     total_ps = sum(r.path_steps for r in reqs)
     ops = sum(r.path_steps * W.valu_ops_per_path_step(r.model, r.steps, r.maturity) for r in reqs)
     chunks = sum(chunks_of(r.paths) for r in reqs)
-    runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks)
+    # autotune the persistent launch shape on a calibration copy of the mix
+    # (same requests, path counts capped at 2^16)
+    from sde_mc_amd.autotune import autotune
+    calib = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps,
+                                     min(r.paths, 1 << 16), r.seed)) for r in reqs]
+    tune, _ = autotune(calib)
+    runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks, tune=tune)
 
     for _ in range(args.warmup):
         ref = runner.run(jobs)
@@ -126,7 +132,7 @@ def main():
         latency = {}
         for rate in (50, 200, 1000):
             arr = W.request_mix(args.requests, seed=args.seed + rate, rate=rate)
-            srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20)
+            srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20, tune=tune)
             res = srv.serve(arr)
             served = latency_summary(res)
             served["rounds"] = srv.rounds
@@ -158,7 +164,10 @@ def main():
         "config": {"workload": "served request mix, %d requests (Merton 1-D / Levy rainbow 2-D), "
                                "%.3g path-steps per pass" % (len(reqs), total_ps),
                    "stepping": "fp32", "sums": "fp64", "grid": "uniform",
-                   "launch": "batched persistent kernel, graph-replayed"},
+                   "launch": "batched persistent kernel, graph-replayed",
+                   "tune": {"waves_per_block": tune.waves_per_block,
+                            "blocks_per_cu": tune.blocks_per_cu,
+                            "tile_chunks": tune.tile_chunks}},
         "plain_value": plain,
         "speedup_vs_plain": (batched / plain) if plain else None,
         "latency": latency,

@@ -246,28 +246,48 @@ typedef struct sdemc_segment {
 /* Bytes of one prepared job slot. */
 size_t sdemc_sizeof_prepared_job(void);
 
+/* Kernel classes of the batched launch: a dedicated persistent kernel for
+ * each of the 1-D 'diag' (Merton, GBM) and 2-D 'indep' (Levy) families on
+ * either grid, single-level or coupled, and one generic kernel that runs
+ * every supported job. */
+#define SDEMC_MAX_GROUPS 8
+enum sdemc_class {
+  SDEMC_CLASS_DIAG1 = 0,  /* + 2 * grid + coupled: classes 0..3 */
+  SDEMC_CLASS_INDEP2 = 4, /* + 2 * grid + coupled: classes 4..7 */
+  SDEMC_CLASS_GENERIC = 8
+};
+
+/* The dedicated class of a job, SDEMC_CLASS_GENERIC, or -1 for an invalid job. */
+int sdemc_job_class(const sdemc_job* job);
+
 /* Validate n host jobs and write their prepared slots (host memory,
  * n * sdemc_sizeof_prepared_job() bytes).  chunk_base[i] is the first chunk
- * partial of job i in the batch partials array.  *class_set receives the
- * smallest kernel class set that covers the jobs (0: 1-D diag and 2-D
- * 'indep' models; 1: every supported model). */
+ * partial of job i in the batch partials array.  classes (nullable)
+ * receives each job's class. */
 int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_base,
-                       void* prepared_out, int32_t* class_set);
+                       void* prepared_out, int32_t* classes);
 
+/* A launch runs n_groups persistent kernels in stream order, one per class
+ * group; group g's segments are segs[hdr[g][0] .. hdr[g][0] + hdr[g][1])
+ * with hdr[g][2] tiles, taken from counter[g].  A segment whose job is not
+ * of the group's class is refused in the kernel (NaN partials, chunk
+ * flag 4) instead of being computed by the wrong code. */
 typedef struct sdemc_batch_dev {
   const void* jobs_dev;          /* [max_jobs] prepared slots                   */
-  const sdemc_segment* segs_dev; /* [max_segments]                              */
-  const int32_t* seg_hdr_dev;    /* [2] {segments, tiles} of this launch (read on
-                                    the device)                                 */
+  const sdemc_segment* segs_dev; /* [max_segments], each group's contiguous     */
+  const int32_t* seg_hdr_dev;    /* [SDEMC_MAX_GROUPS][4] {first segment,
+                                    segments, tiles, 0}, read on the device     */
   const int32_t* finalize_dev;   /* [max_jobs] nonzero: finalise slot this launch */
   double* partials_dev;          /* [partial_chunks][2]                         */
-  int32_t* chunk_flags_dev;      /* [partial_chunks] overflow flags             */
+  int32_t* chunk_flags_dev;      /* [partial_chunks] flags                      */
   double* stats_dev;             /* [max_jobs][4]                               */
-  int32_t* counter_dev;          /* [1] tile counter, zeroed by the launch      */
+  int32_t* counter_dev;          /* [SDEMC_MAX_GROUPS] tile counters, zeroed by
+                                    the launch                                  */
   int64_t partial_chunks;        /* capacity of partials / chunk_flags          */
   int32_t max_jobs;
   int32_t max_segments;
-  int32_t class_set;             /* from sdemc_prepare_jobs                     */
+  int32_t n_groups;              /* 1..SDEMC_MAX_GROUPS                         */
+  int32_t group_class[SDEMC_MAX_GROUPS]; /* enum sdemc_class of each group      */
   int32_t reserved;
 } sdemc_batch_dev;
 
@@ -278,11 +298,11 @@ typedef struct sdemc_batch_tune {
   int32_t reserved1;
 } sdemc_batch_tune;
 
-/* One batched launch: zero the tile counter, run the persistent tile kernel,
+/* One batched launch: zero the tile counters, run the group kernels,
  * finalise the flagged slots.  Everything is stream-ordered. */
 int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream);
 
-/* The same three launches captured once into a HIP graph and replayed by
+/* The same launches captured once into a HIP graph and replayed by
  * sdemc_batch_plan_launch; the graph reads the segment header and the
  * tables from the device, so one plan serves every launch that uses the
  * same buffers. */

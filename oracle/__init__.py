@@ -42,6 +42,14 @@ def load():
         lib.oracle_simulate_fixed.restype = ctypes.c_int
         lib.oracle_levy_icdf.argtypes = [ctypes.c_void_p, ctypes.c_float]
         lib.oracle_levy_icdf.restype = ctypes.c_double
+        lib.oracle_ln_unit.argtypes = [ctypes.c_uint32]
+        lib.oracle_ln_unit.restype = ctypes.c_float
+        lib.oracle_poisson_thresholds.argtypes = [ctypes.c_double, ctypes.c_void_p,
+                                                  ctypes.c_void_p]
+        lib.oracle_poisson_thresholds.restype = ctypes.c_int
+        lib.oracle_step_jumps.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                                          ctypes.c_void_p]
+        lib.oracle_step_jumps.restype = ctypes.c_int
         _LIB = lib
     return _LIB
 
@@ -93,3 +101,22 @@ def simulate_fixed(job, fixed, P):
 
 def levy_icdf(job, y):
     return load().oracle_levy_icdf(ctypes.addressof(job), float(np.float32(y)))
+
+
+def ln_unit(w):
+    """fp32 ln((w | 1) 2^-32) as the kernels compute the arrival gaps."""
+    return load().oracle_ln_unit(int(w) & 0xFFFFFFFF)
+
+
+def poisson_thresholds(mu):
+    T = (ctypes.c_uint32 * 16)()
+    inv = ctypes.c_double()
+    ok = load().oracle_poisson_thresholds(float(mu), T, ctypes.byref(inv))
+    return [int(v) for v in T], inv.value, bool(ok)
+
+
+def step_jumps(job, path, n):
+    """(count, first mark) of uniform-grid step n in per-step Poisson mode."""
+    fm = ctypes.c_double()
+    c = load().oracle_step_jumps(ctypes.addressof(job), int(path), int(n), ctypes.byref(fm))
+    return c, fm.value

@@ -67,6 +67,53 @@ static void philox_block(uint32_t blk, uint32_t stream, uint64_t path, uint64_t 
 static double u_open(uint32_t w) { return (double)((w >> 8) | 1u) * 0x1p-24; }
 static double u_half(uint32_t w) { return (double)(w >> 8) * 0x1p-24; }
 
+/* ln u for u = (w | 1) 2^-32, fp32: u = m 2^e, m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(s) with s = (m-1)/(m+1) by the odd series to s^9.  Every
+ * operation is a single correctly rounded fp32 operation (this file is built
+ * with -ffp-contract=off), which is what makes the exponential arrival gaps
+ * of the kernels reproducible here bit for bit. */
+ORACLE_API float oracle_ln_unit(uint32_t w) {
+  uint32_t v = w | 1u;
+  int z = __builtin_clz(v);
+  uint32_t top = v << z;
+  float m = (float)(top >> 8) * 0x1p-23f;
+  int e = -(z + 1);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e = e + 1;
+  }
+  float sn = m - 1.0f, sd = m + 1.0f;
+  float sq = sn / sd;
+  float s2 = sq * sq;
+  float p = fmaf(s2, 0.222222222f, 0.285714286f);
+  p = fmaf(s2, p, 0.4f);
+  p = fmaf(s2, p, 0.666666667f);
+  p = fmaf(s2, p, 2.0f);
+  float lnm = sq * p;
+  float et = (float)e * 0.693147181f;
+  return lnm + et;
+}
+
+/* P(N >= k) 2^32, k = 1..16, of N ~ Poisson(mu), rounded to integers; the
+ * tail sums run from the top.  Returns whether P(N > 16) < 2^-33. */
+static int poisson_thresholds(double mu, uint32_t T16[16], double* inv) {
+  double p[64], tail[65];
+  p[0] = exp(-mu);
+  for (int i = 1; i < 64; ++i) p[i] = p[i - 1] * mu / (double)i;
+  tail[64] = 0.0;
+  for (int i = 63; i >= 0; --i) tail[i] = tail[i + 1] + p[i];
+  for (int k = 1; k <= 16; ++k) {
+    double v = tail[k] * 4294967296.0;
+    T16[k - 1] = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(v + 0.5);
+  }
+  *inv = T16[0] ? 1.0 / (double)T16[0] : 0.0;
+  return tail[17] * 4294967296.0 < 0.5;
+}
+
+ORACLE_API int oracle_poisson_thresholds(double mu, uint32_t* T16, double* inv) {
+  return poisson_thresholds(mu, T16, inv);
+}
+
 /* normal i of a path: word i%4 of Brownian block i/4, Box-Muller pairs (0,1),(2,3) */
 ORACLE_API double oracle_normal(uint64_t seed, uint64_t path, uint32_t i) {
   uint32_t w[4];
@@ -84,6 +131,10 @@ typedef struct {
   int D, K, noise;
   double L[D_MAX][D_MAX];
   float lv_y1, lv_y2, lv_y3;
+  float inv_rate, n_over_T;
+  int pois;
+  uint32_t pT[16];
+  double pinv;
 } model_t;
 
 static void model_init(model_t* m, const sdemc_job* job) {
@@ -96,6 +147,17 @@ static void model_init(model_t* m, const sdemc_job* job) {
   m->lv_y1 = (float)job->model.jp[6];
   m->lv_y2 = (float)job->model.jp[7];
   m->lv_y3 = (float)job->model.jp[8];
+  m->inv_rate = job->model.rate > 0.0 ? (float)(1.0 / job->model.rate) : 0.0f;
+  m->n_over_T = (float)((double)job->steps / job->T);
+  m->pois = 0;
+  memset(m->pT, 0, sizeof(m->pT));
+  m->pinv = 0.0;
+  /* per-step Poisson counts: Levy marks on the uniform grid */
+  if (job->model.jump_law == SDEMC_JUMP_LEVY && job->grid == SDEMC_GRID_UNIFORM &&
+      job->model.rate > 0.0) {
+    double mu = job->model.rate * job->T / (double)job->steps;
+    m->pois = poisson_thresholds(mu, m->pT, &m->pinv);
+  }
 }
 
 /* InverseCdf.__call__ (levy.py:22-30) in fp64, branch chosen on fp32 y */
@@ -211,26 +273,26 @@ typedef struct {
   const float* jump_size;
   int64_t n_marks, off;
   int32_t n_rows, max_jumps;
-  /* in-kernel jump clock */
-  double tau;
+  /* in-kernel jump clock: fp32 arrival time accumulated gap by gap */
+  float tau;
   uint32_t j, m1, m2;
-  double inv_rate;
+  float inv_rate;
   int bad;
 } src_t;
 
 static void clock_draw(src_t* s) {
   uint32_t w[4];
   philox_block(s->j, 1, s->path, s->seed, w);
-  double u = ((double)w[0] + 0.5) * 0x1p-32;
-  s->tau += -log(u) * s->inv_rate;
+  float gap = -oracle_ln_unit(w[0]) * s->inv_rate;
+  s->tau = s->tau + gap;
   s->m1 = w[1];
   s->m2 = w[2];
 }
 
 static void clock_init(src_t* s, const sdemc_job* job) {
   s->j = 0;
-  s->tau = 0.0;
-  s->inv_rate = job->model.rate > 0.0 ? 1.0 / job->model.rate : 0.0;
+  s->tau = 0.0f;
+  s->inv_rate = job->model.rate > 0.0 ? (float)(1.0 / job->model.rate) : 0.0f;
   if (job->model.rate > 0.0)
     clock_draw(s);
   else
@@ -238,9 +300,36 @@ static void clock_init(src_t* s, const sdemc_job* job) {
 }
 
 /* step index (grid of n steps on [0,T]) of time tau */
-static int32_t step_of(double tau, double n_over_T, int32_t n) {
-  double q = tau * n_over_T;
-  return q < (double)n ? (int32_t)q : 0x7fffffff;
+static int32_t step_of(float tau, float n_over_T, int32_t n) {
+  float q = tau * n_over_T;
+  return q < (float)n ? (int32_t)q : 0x7fffffff;
+}
+
+/* marks of step n under per-step Poisson counts: count = #{k: w < T_k};
+ * mark 0 from the residual (w + 1/2) / T_1, mark k >= 1 from word (k-1) % 4
+ * of mark-stream block (n, 3 + 16 ((k-1) / 4)); returns the number of marks */
+static int poisson_marks(const model_t* m, src_t* s, int32_t n, double* marks, int cap) {
+  uint32_t pw[4];
+  philox_block((uint32_t)n >> 2, 2, s->path, s->seed, pw);
+  uint32_t w = pw[n & 3];
+  if (!(w < m->pT[0])) return 0;
+  /* count c: T_{c+1} <= w < T_c; the offset within that interval is the
+   * first mark's uniform, independent of c */
+  int c = 1;
+  while (c < 16 && w < m->pT[c]) ++c;
+  uint32_t lo = c < 16 ? m->pT[c] : 0u, hi = m->pT[c - 1];
+  double r = ((double)(w - lo) + 0.5) / (double)(hi - lo);
+  float y = (float)r;
+  if (y > 0x1.fffffep-1f) y = 0x1.fffffep-1f;
+  for (int k = 0; k < c; ++k) {
+    if (k > 0) {
+      uint32_t q = (uint32_t)(k - 1), mw[4];
+      philox_block((uint32_t)n, 3u + 16u * (q >> 2), s->path, s->seed, mw);
+      y = (float)u_open(mw[q & 3u]);
+    }
+    if (k < cap) marks[k] = levy_icdf(m, y);
+  }
+  return c;
 }
 
 /* fixed-increment row n (uniform grid): D rows, then D+1-th (common) or D more */
@@ -263,6 +352,21 @@ static double fixed_mark(src_t* s) {
   return J;
 }
 
+/* jump count and first mark of step n of path `path` in per-step Poisson mode */
+ORACLE_API int oracle_step_jumps(const sdemc_job* job, uint64_t path, int32_t n, double* first_mark) {
+  model_t m;
+  model_init(&m, job);
+  if (!m.pois) return -1;
+  src_t s;
+  memset(&s, 0, sizeof(s));
+  s.seed = job->seed;
+  s.path = path;
+  double marks[16];
+  int c = poisson_marks(&m, &s, n, marks, 16);
+  if (first_mark) *first_mark = c > 0 ? marks[0] : 0.0;
+  return c;
+}
+
 /* ---------------- uniform grid ----------------------------------------- */
 /* Single level: N steps of h = T/N; the jumps of step n (those with arrival
  * in [nh, (n+1)h)) are added at its end with the coefficient at the
@@ -271,11 +375,10 @@ static void uniform_single(const model_t* m, src_t* s, double* x) {
   const sdemc_job* job = m->job;
   int D = m->D, K = m->K, N = job->steps;
   double h = job->T / N, sq = sqrt(h);
-  double n_over_T = (double)N / job->T;
   int32_t kj = 0;
-  if (!s->fixed) {
+  if (!s->fixed && !m->pois) {
     clock_init(s, job);
-    kj = step_of(s->tau, n_over_T, N);
+    kj = step_of(s->tau, m->n_over_T, N);
   }
   for (int i = 0; i < D; ++i) x[i] = job->model.x0[i];
   for (int32_t n = 0; n < N; ++n) {
@@ -294,6 +397,10 @@ static void uniform_single(const model_t* m, src_t* s, double* x) {
         double J = fixed_mark(s);
         add_jump(m, x, job->model.exact_jumps ? x : xo, J);
       }
+    } else if (m->pois) {
+      double marks[16];
+      int c = poisson_marks(m, s, n, marks, 16);
+      for (int k = 0; k < c; ++k) add_jump(m, x, job->model.exact_jumps ? x : xo, marks[k]);
     } else {
       while (kj == n) {
         double J = mark_from_words(m, s->m1, s->m2);
@@ -302,7 +409,7 @@ static void uniform_single(const model_t* m, src_t* s, double* x) {
         add_jump(m, x, y, J);
         s->j++;
         clock_draw(s);
-        kj = step_of(s->tau, n_over_T, N);
+        kj = step_of(s->tau, m->n_over_T, N);
       }
     }
   }
@@ -316,11 +423,10 @@ static void uniform_coupled(const model_t* m, src_t* s, double* xf, double* xc) 
   const sdemc_job* job = m->job;
   int D = m->D, K = m->K, N = job->steps, M = job->coarse_factor;
   double hf = job->T / N, hc = job->T * M / N, sq = sqrt(hf);
-  double n_over_T = (double)N / job->T;
   int32_t kj = 0;
-  if (!s->fixed) {
+  if (!s->fixed && !m->pois) {
     clock_init(s, job);
-    kj = step_of(s->tau, n_over_T, N);
+    kj = step_of(s->tau, m->n_over_T, N);
   }
   for (int i = 0; i < D; ++i) xf[i] = xc[i] = job->model.x0[i];
   double* marks = NULL;
@@ -344,19 +450,25 @@ static void uniform_coupled(const model_t* m, src_t* s, double* xf, double* xc) 
       memcpy(xo, xf, sizeof(double) * D);
       euler(m, xf, hf, dW, dB);
       int c;
+      double pm[16];
+      int pk = 0;
       if (s->fixed) c = s->counts ? s->counts[(int64_t)n * s->P + s->p] : 0;
+      else if (m->pois) c = poisson_marks(m, s, n, pm, 16);
       else c = 0;
       for (;;) {
         double J;
         if (s->fixed) {
           if (c-- <= 0) break;
           J = fixed_mark(s);
+        } else if (m->pois) {
+          if (pk >= c) break;
+          J = pm[pk++];
         } else {
           if (kj != n) break;
           J = mark_from_words(m, s->m1, s->m2);
           s->j++;
           clock_draw(s);
-          kj = step_of(s->tau, n_over_T, N);
+          kj = step_of(s->tau, m->n_over_T, N);
         }
         add_jump(m, xf, job->model.exact_jumps ? xf : xo, J);
         if (nm == cap) {
@@ -384,7 +496,7 @@ static int isclose32(float tau, float t) {
 
 static float next_time(src_t* s) {
   if (s->fixed) return s->j < (uint32_t)s->max_jumps ? s->jump_time[(int64_t)s->j * s->P + s->p] : INFINITY;
-  return (float)s->tau;
+  return s->tau;
 }
 
 static double take_mark(const model_t* m, src_t* s) {

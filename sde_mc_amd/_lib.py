@@ -65,6 +65,8 @@ class FixedIn(ctypes.Structure):
 
 
 SEGMENT_INTS = 6  # sdemc_segment: job, first, count, per_tile, tile_begin, reserved
+MAX_GROUPS = 8    # class groups per batched launch
+CLASS_GENERIC = 8  # dedicated classes are 0..7: 4 * (dim-2 family) + 2 * grid + coupled
 
 
 class BatchDev(ctypes.Structure):
@@ -74,8 +76,8 @@ class BatchDev(ctypes.Structure):
         ("partials_dev", ctypes.c_void_p), ("chunk_flags_dev", ctypes.c_void_p),
         ("stats_dev", ctypes.c_void_p), ("counter_dev", ctypes.c_void_p),
         ("partial_chunks", ctypes.c_int64), ("max_jobs", ctypes.c_int32),
-        ("max_segments", ctypes.c_int32), ("class_set", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("max_segments", ctypes.c_int32), ("n_groups", ctypes.c_int32),
+        ("group_class", ctypes.c_int32 * MAX_GROUPS), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -124,6 +126,7 @@ _SIGS = {
     "sdemc_reduce_finalize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                              ctypes.c_void_p]),
+    "sdemc_job_class": (ctypes.c_int, [ctypes.POINTER(Job)]),
     "sdemc_prepare_jobs": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int32)]),

@@ -1,0 +1,82 @@
+"""Autotuner of the batched persistent launch.
+
+Sweeps the launch shape -- wavefronts per workgroup (so paths per workgroup
+= 64 x wavefronts), resident workgroups per CU and chunks per tile -- on a
+calibration batch, timing the tile kernel with device events, and keeps the
+fastest shape per (device, set of kernel classes in the batch).  Results are cached in a JSON
+file next to the library so later processes start tuned.
+"""
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .batch import BatchRunner, BatchTune, chunks_of, job_class
+
+CANDIDATES = [BatchTune(w, b, t) for w in (2, 4, 8) for b in (2, 4, 8, 16) for t in (1, 2, 4)
+              if 2 <= w * b <= 64]
+
+
+def cache_path():
+    return os.environ.get('SDEMC_TUNE_CACHE',
+                          os.path.join(os.path.dirname(L.lib_path()), 'autotune.json'))
+
+
+def _device_key():
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return '%s/%d' % (getattr(p, 'gcnArchName', p.name), p.multi_processor_count)
+
+
+def load_cache():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(c):
+    try:
+        with open(cache_path(), 'w') as f:
+            json.dump(c, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def time_tune(runner, jobs, tune, reps=2):
+    """Mean tile-kernel time (ms) of `jobs` in one batched launch with `tune`."""
+    runner.run(jobs, tune=tune)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for e in ev:
+        runner.run(jobs, tune=tune, events=e)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+
+def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False):
+    """Best BatchTune for batches like `jobs`; returns (tune, table of timings)."""
+    classes = sorted(set(job_class(j) for j in jobs))
+    key = '%s/classes%s' % (_device_key(), '-'.join(str(c) for c in classes))
+    cache = load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        return BatchTune(*cache[key]['best']), cache[key]['table']
+    counts = sum(chunks_of(j.n_paths) for j in jobs)
+    runner = BatchRunner(max_jobs=max(8, len(jobs)), partial_chunks=counts)
+    table = []
+    t0 = time.time()
+    for tune in candidates or CANDIDATES:
+        ms = time_tune(runner, jobs, tune, reps)
+        table.append((tune.key(), ms))
+        if verbose:
+            print('autotune %r: %.3f ms' % (tune, ms), flush=True)
+    runner.close()
+    best = min(table, key=lambda r: r[1])
+    cache = load_cache()
+    cache[key] = {'best': list(best[0]), 'table': table, 'seconds': time.time() - t0}
+    if use_cache:
+        save_cache(cache)
+    return BatchTune(*best[0]), table

@@ -4,12 +4,15 @@ models, MLMC levels) in one kernel launch.
 A :class:`BatchRunner` owns a device job table (prepared slots), a pool of
 chunk partials, per-slot statistics and one or more round buffers.  A round
 buffer holds a segment list -- (slot, first chunk, chunks, chunks per tile,
-first tile) -- and the mask of slots to finalise; the persistent kernel's
-wavefronts take tile indices from a device counter until the round's tiles
-are exhausted.  The three launches of a round (counter reset, tile kernel,
-finalisation) are captured once per round buffer in a HIP graph and
-replayed; the host only uploads the small segment list and, for new jobs,
-their slots.
+first tile) -- grouped by kernel class, and the mask of slots to finalise.
+Each class group runs in its own persistent kernel, specialised to one
+(model family, time grid, single level or coupled pair) exactly like the
+plain launch's kernel for it, or the generic kernel for anything else; its
+wavefronts take tile indices from the group's device counter until the
+group's tiles are exhausted.  The launches of a round (counter reset, the
+group kernels, finalisation) are captured once per (round buffer, class
+groups) in a HIP graph and replayed; the host only uploads the small
+segment list and, for new jobs, their slots.
 
 A job's result is independent of what it is packed with or how it is cut:
 the Philox streams are keyed by (seed, path) and the reduction order is
@@ -57,6 +60,16 @@ def job_cost(job):
     return c
 
 
+def job_class(job):
+    """Kernel class of a job (sdemc_job_class): 0..7 are the dedicated kernels
+    of the 1-D 'diag' (0..3) and 2-D 'indep' (4..7) families, 2 * grid +
+    coupled within each; 8 is the generic kernel that runs every job."""
+    c = L.lib().sdemc_job_class(job)
+    if c < 0:
+        raise L.SdemcError(L.ERR_ARG, 'sdemc_job_class')
+    return int(c)
+
+
 class BatchTune:
     """Launch shape of the persistent kernel: wavefronts per workgroup,
     resident workgroups per CU, chunks (64 paths each) per tile."""
@@ -79,31 +92,70 @@ class BatchTune:
         return 'BatchTune(wpb=%d, bpc=%d, tile_chunks=%d)' % self.key()
 
 
-def plan_segments(slots, firsts, counts, costs, tile_chunks):
-    """Segment table [n, 6] ordered costliest chunk first, with consecutive
-    tile ranges; returns (segments, total tiles)."""
+class RoundPlan:
+    """One round's segment table grouped by kernel class.
+
+    segs   [n, 6] int32: (slot, first chunk, chunks, chunks per tile, first
+           tile within its group, 0), each group's rows contiguous and
+           costliest chunk first;
+    hdr    [MAX_GROUPS, 4] int32: (first segment, segments, tiles, 0);
+    groups the kernel class of each group, in launch order;
+    tiles  tiles over all groups."""
+
+    __slots__ = ('segs', 'hdr', 'groups', 'tiles')
+
+    def __init__(self, segs, hdr, groups, tiles):
+        self.segs, self.hdr, self.groups, self.tiles = segs, hdr, tuple(groups), int(tiles)
+
+
+def plan_segments(slots, firsts, counts, costs, tile_chunks, classes=None):
+    """Group the segments by kernel class (generic when `classes` is None) and
+    lay out consecutive tile ranges within each group; returns a RoundPlan.
+    Groups launch one after another, the costliest group first; when more
+    classes are present than a launch has groups, the cheapest dedicated
+    groups fold into the generic one."""
     slots = np.asarray(slots, dtype=np.int64)
     firsts = np.asarray(firsts, dtype=np.int64)
     counts = np.asarray(counts, dtype=np.int64)
+    costs = np.asarray(costs, dtype=np.float64)
+    if classes is None:
+        cls = np.full(slots.size, L.CLASS_GENERIC, dtype=np.int64)
+    else:
+        cls = np.asarray(classes, dtype=np.int64).copy()
     keep = counts > 0
-    slots, firsts, counts = slots[keep], firsts[keep], counts[keep]
-    costs = np.asarray(costs, dtype=np.float64)[keep]
-    order = np.argsort(-costs, kind='stable')
+    slots, firsts, counts, costs, cls = slots[keep], firsts[keep], counts[keep], costs[keep], cls[keep]
+    if np.any((cls < 0) | (cls > L.CLASS_GENERIC)):
+        raise ValueError('kernel class out of range')
+    work = {int(c): float((counts[cls == c] * costs[cls == c]).sum()) for c in np.unique(cls)}
+    while len(work) > L.MAX_GROUPS:
+        c = min((k for k in work if k != L.CLASS_GENERIC), key=lambda k: work[k])
+        cls[cls == c] = L.CLASS_GENERIC
+        work[L.CLASS_GENERIC] = work.get(L.CLASS_GENERIC, 0.0) + work.pop(c)
+    groups = sorted(work, key=lambda k: (-work[k], k))
+    rank = np.zeros(L.CLASS_GENERIC + 1, dtype=np.int64)
+    for g, c in enumerate(groups):
+        rank[c] = g
+    order = np.lexsort((-costs, rank[cls])) if slots.size else np.zeros(0, np.int64)
     per = np.full(order.size, max(1, int(tile_chunks)), dtype=np.int64)
     nt = (counts[order] + per - 1) // per
+    grp = rank[cls[order]]
+    hdr = np.zeros((L.MAX_GROUPS, 4), dtype=np.int32)
     begin = np.zeros(order.size, dtype=np.int64)
-    if order.size:
-        begin[1:] = np.cumsum(nt)[:-1]
+    for g in range(len(groups)):
+        idx = np.nonzero(grp == g)[0]
+        t = nt[idx]
+        begin[idx[1:]] = np.cumsum(t)[:-1]
+        total = int(t.sum())
+        if total >= 2 ** 31:
+            raise ValueError('too many tiles in one launch')
+        hdr[g] = (idx[0], idx.size, total, 0)
     seg = np.zeros((order.size, NS), dtype=np.int32)
     seg[:, 0] = slots[order]
     seg[:, 1] = firsts[order]
     seg[:, 2] = counts[order]
     seg[:, 3] = per
     seg[:, 4] = begin
-    total = int(nt.sum()) if order.size else 0
-    if total >= 2 ** 31:
-        raise ValueError('too many tiles in one launch')
-    return seg, total
+    return RoundPlan(seg, hdr, groups, int(nt.sum()) if order.size else 0)
 
 
 class RoundBuffer:
@@ -113,10 +165,10 @@ class RoundBuffer:
     def __init__(self, device, max_jobs, max_segments):
         self.max_segments = int(max_segments)
         self.segs = torch.zeros(self.max_segments * NS, dtype=torch.int32, device=device)
-        self.hdr = torch.zeros(2, dtype=torch.int32, device=device)
+        self.hdr = torch.zeros(4 * L.MAX_GROUPS, dtype=torch.int32, device=device)
         self.final = torch.zeros(max_jobs, dtype=torch.int32, device=device)
         self.h_segs = torch.zeros(self.max_segments * NS, dtype=torch.int32, pin_memory=True)
-        self.h_hdr = torch.zeros(2, dtype=torch.int32, pin_memory=True)
+        self.h_hdr = torch.zeros(4 * L.MAX_GROUPS, dtype=torch.int32, pin_memory=True)
         self.h_final = torch.zeros(max_jobs, dtype=torch.int32, pin_memory=True)
         self.h_stats = torch.zeros(max_jobs * 4, dtype=torch.float64, pin_memory=True)
         self.done = None  # event recorded after the round's read-back
@@ -138,7 +190,8 @@ class BatchRunner:
         self.jobs = torch.zeros(self.max_jobs * self.kb, dtype=torch.uint8, device=dev)
         self.h_jobs = torch.zeros(self.max_jobs * self.kb, dtype=torch.uint8, pin_memory=True)
         self.stats = torch.zeros(self.max_jobs * 4, dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.counter = torch.zeros(L.MAX_GROUPS, dtype=torch.int32, device=dev)
+        self.slot_class = np.full(self.max_jobs, L.CLASS_GENERIC, dtype=np.int32)
         self.bufs = [RoundBuffer(dev, self.max_jobs, self.max_segments) for _ in range(n_buffers)]
         self._plans = {}
         self._alloc_partials(partial_chunks)
@@ -160,7 +213,7 @@ class BatchRunner:
         except Exception:
             pass
 
-    def batch_dev(self, buf, class_set):
+    def batch_dev(self, buf, groups):
         b = L.BatchDev()
         b.jobs_dev = self.jobs.data_ptr()
         b.segs_dev = buf.segs.data_ptr()
@@ -173,26 +226,32 @@ class BatchRunner:
         b.partial_chunks = self.partial_chunks
         b.max_jobs = self.max_jobs
         b.max_segments = buf.max_segments
-        b.class_set = int(class_set)
+        if not 1 <= len(groups) <= L.MAX_GROUPS:
+            raise ValueError('a launch runs 1..%d class groups' % L.MAX_GROUPS)
+        b.n_groups = len(groups)
+        for g, c in enumerate(groups):
+            b.group_class[g] = int(c)
         return b
 
     # -- tables ----------------------------------------------------------
     def prepare(self, jobs, chunk_base, slots):
         """Validate jobs and stage their prepared bytes into slots (pinned host
-        table); returns the class set they need."""
+        table); returns their kernel classes."""
         n = len(jobs)
         if n == 0:
-            return 0
+            return np.zeros(0, dtype=np.int32)
         arr = (L.Job * n)(*jobs)
         base = (ctypes.c_int64 * n)(*[int(b) for b in chunk_base])
         buf = ctypes.create_string_buffer(n * self.kb)
-        cs = ctypes.c_int32(0)
-        L.check(self.lib.sdemc_prepare_jobs(arr, n, base, buf, ctypes.byref(cs)),
-                'sdemc_prepare_jobs')
+        cls = (ctypes.c_int32 * n)()
+        L.check(self.lib.sdemc_prepare_jobs(arr, n, base, buf, cls), 'sdemc_prepare_jobs')
         raw = np.frombuffer(buf.raw, dtype=np.uint8).reshape(n, self.kb)
         hj = self.h_jobs.numpy().reshape(self.max_jobs, self.kb)
-        hj[np.asarray(slots, dtype=np.int64)] = raw
-        return int(cs.value)
+        sl = np.asarray(slots, dtype=np.int64)
+        hj[sl] = raw
+        out = np.frombuffer(cls, dtype=np.int32).copy()
+        self.slot_class[sl] = out
+        return out
 
     def upload_slots(self, slots=None):
         """H2D copy of staged slots (all when None), ordered on the current stream."""
@@ -209,15 +268,15 @@ class BatchRunner:
             self.jobs[a:b].copy_(self.h_jobs[a:b], non_blocking=True)
             i = j + 1
 
-    def stage_round(self, buf, segs, total_tiles, finalize_slots):
+    def stage_round(self, buf, plan, finalize_slots):
+        segs = plan.segs
         n = len(segs)
         if n > buf.max_segments:
             raise ValueError('%d segments exceed max_segments %d' % (n, buf.max_segments))
         if n:
             buf.h_segs.numpy()[:NS * n] = segs.reshape(-1)
             buf.segs[:NS * n].copy_(buf.h_segs[:NS * n], non_blocking=True)
-        buf.h_hdr[0] = n
-        buf.h_hdr[1] = total_tiles
+        buf.h_hdr.numpy()[:] = plan.hdr.reshape(-1)
         buf.hdr.copy_(buf.h_hdr, non_blocking=True)
         hf = buf.h_final.numpy()
         hf[:] = 0
@@ -227,23 +286,26 @@ class BatchRunner:
         buf.final.copy_(buf.h_final, non_blocking=True)
 
     # -- launch ----------------------------------------------------------
-    def launch(self, buf, class_set=1, tune=None):
+    def launch(self, buf, groups, tune=None):
+        """Launch the round staged in `buf`: one persistent kernel per class
+        group in `groups` (a RoundPlan's groups), then the finalisation."""
         tune = tune or self.tune
+        groups = tuple(int(g) for g in groups) or (L.CLASS_GENERIC,)
         stream = E.stream_ptr()
         if self.use_graph:
-            key = (id(buf), class_set, tune.waves_per_block, tune.blocks_per_cu,
+            key = (id(buf), groups, tune.waves_per_block, tune.blocks_per_cu,
                    self.partials.data_ptr())
             plan = self._plans.get(key)
             if plan is None:
                 h = ctypes.c_void_p()
                 L.check(self.lib.sdemc_batch_plan_create(
-                    ctypes.byref(self.batch_dev(buf, class_set)), ctypes.byref(tune.c()),
+                    ctypes.byref(self.batch_dev(buf, groups)), ctypes.byref(tune.c()),
                     ctypes.byref(h)), 'sdemc_batch_plan_create')
                 plan = h
                 self._plans[key] = plan
             L.check(self.lib.sdemc_batch_plan_launch(plan, stream), 'sdemc_batch_plan_launch')
         else:
-            L.check(self.lib.sdemc_batch_launch(ctypes.byref(self.batch_dev(buf, class_set)),
+            L.check(self.lib.sdemc_batch_launch(ctypes.byref(self.batch_dev(buf, groups)),
                                                 ctypes.byref(tune.c()), stream),
                     'sdemc_batch_launch')
 
@@ -271,15 +333,15 @@ class BatchRunner:
         if total > self.partial_chunks:
             self._alloc_partials(max(total, 2 * self.partial_chunks))
         slots = np.arange(n)
-        cs = self.prepare(jobs, base, slots)
+        cls = self.prepare(jobs, base, slots)
         self.upload_slots(range(n))
         costs = np.array([job_cost(j) for j in jobs])
-        segs, ntiles = plan_segments(slots, np.zeros(n, np.int64), counts, costs, tune.tile_chunks)
+        plan = plan_segments(slots, np.zeros(n, np.int64), counts, costs, tune.tile_chunks, cls)
         buf = self.bufs[0]
-        self.stage_round(buf, segs, ntiles, slots)
+        self.stage_round(buf, plan, slots)
         if events is not None:
             events[0].record()
-        self.launch(buf, cs, tune)
+        self.launch(buf, plan.groups, tune)
         if events is not None:
             events[1].record()
         buf.h_stats[:4 * n].copy_(self.stats[:4 * n], non_blocking=True)

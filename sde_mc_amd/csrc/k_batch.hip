@@ -57,6 +57,16 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
   chunk_store(dv, dv * dv, ovf, part, cf);
 }
 
+// Class of a job: dedicated kernels for the 1-D 'diag' and 2-D 'indep'
+// families (grid and level type folded in), else the generic kernel.
+inline int32_t job_class_of(int noise, int dim, int grid, bool coupled) {
+  int32_t fam = -1;
+  if (noise == SDEMC_NOISE_DIAG && dim == 1) fam = 0;
+  if (noise == SDEMC_NOISE_INDEP_COMMON && dim == 2) fam = 1;
+  if (fam < 0) return SDEMC_CLASS_GENERIC;
+  return fam * 4 + grid * 2 + (coupled ? 1 : 0);
+}
+
 #define SDEMC_CASE(NOISE, D, FAM, GRID, C)                              \
   case ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + C:                      \
     chunk_body<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);           \
@@ -66,19 +76,21 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
   SDEMC_CASE(NOISE, D, FAM, GRID, 0)        \
   SDEMC_CASE(NOISE, D, FAM, GRID, 1)
 
-// Class set 0: the 1-D 'diag' family (Merton, GBM) and the 2-D 'indep'
-// family (Levy rainbow); set 1: every supported (noise, dim, grid, level).
-template <int SET>
+// One chunk of a job in kernel class CLS: the dedicated classes run exactly
+// one (family, grid, level type) -- the code of the plain kernel for it --
+// and refuse anything else; the generic class runs every supported job.
+template <int CLS>
 __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double* part,
                                           int32_t* cf) {
-  if (SET == 0) {
-    switch (job.class_key) {
-      SDEMC_CASES_DG(0, 1, FAM_MERTON, 0)
-      SDEMC_CASES_DG(0, 1, FAM_MERTON, 1)
-      SDEMC_CASES_DG(1, 2, FAM_LEVY, 0)
-      SDEMC_CASES_DG(1, 2, FAM_LEVY, 1)
-      default: return false;
-    }
+  if (CLS < SDEMC_CLASS_GENERIC) {
+    constexpr int FAMI = CLS / 4, GRID = (CLS / 2) % 2;
+    constexpr bool COUPLED = (CLS % 2) != 0;
+    constexpr int NOISE = FAMI == 0 ? SDEMC_NOISE_DIAG : SDEMC_NOISE_INDEP_COMMON;
+    constexpr int D = FAMI == 0 ? 1 : 2;
+    constexpr int FAM = FAMI == 0 ? FAM_MERTON : FAM_LEVY;
+    if (job.class_key != ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + (COUPLED ? 1 : 0)) return false;
+    chunk_body<D, FAM, GRID, COUPLED>(job, chunk, part, cf);
+    return true;
   } else {
     switch (job.class_key) {
       SDEMC_CASES_DG(0, 1, FAM_MERTON, 0)
@@ -109,7 +121,9 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
 #undef SDEMC_CASE
 #undef SDEMC_CASES_DG
 
-template <int SET>
+// Persistent tile kernel of one class group.  hdr = {first segment,
+// segments, tiles}; tile indices come from this group's counter.
+template <int CLS>
 __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
                                                const sdemc_segment* __restrict__ segs,
                                                const int32_t* __restrict__ hdr,
@@ -119,11 +133,13 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
                                                int64_t partial_chunks, int32_t max_jobs,
                                                int32_t max_segments) {
   const int lane = threadIdx.x & (WAVE - 1);
-  int32_t nseg = hdr[0];
-  nseg = nseg < max_segments ? nseg : max_segments;
-  const int32_t ntiles = hdr[1];
-  if (nseg <= 0) return;
-  int32_t seg = 0;
+  const int32_t s0 = hdr[0];
+  int32_t nseg = hdr[1];
+  const int32_t ntiles = hdr[2];
+  if (s0 < 0 || s0 >= max_segments || nseg <= 0) return;
+  nseg = nseg < max_segments - s0 ? nseg : max_segments - s0;
+  int32_t seg = s0;
+  const int32_t last = s0 + nseg - 1;
   for (;;) {
     int32_t t = 0;
     if (lane == 0) t = atomicAdd(counter, 1);
@@ -131,8 +147,8 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
     t = __builtin_amdgcn_readfirstlane(t);
     if (t >= ntiles) break;
     // segment owning tile t: the last one whose tile_begin <= t; a wave's
-    // tile indices only grow, so search [seg, nseg)
-    int32_t lo = seg, hi = nseg - 1;
+    // tile indices only grow, so search [seg, last]
+    int32_t lo = seg, hi = last;
     while (lo < hi) {
       const int32_t mid = (lo + hi + 1) >> 1;
       if (segs[mid].tile_begin <= t) lo = mid;
@@ -153,13 +169,18 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
     for (int64_t chunk = c0; chunk < c1; ++chunk) {
       const int64_t slot = job.chunk_base + chunk;
       if (slot < 0 || slot >= partial_chunks) break;
-      run_chunk<SET>(job, chunk, partials + 2 * slot, cflags + slot);
+      if (!run_chunk<CLS>(job, chunk, partials + 2 * slot, cflags + slot) && lane == 0) {
+        // a job outside this kernel's class: poison its partial and flag it
+        partials[2 * slot] = __builtin_nan("");
+        partials[2 * slot + 1] = __builtin_nan("");
+        cflags[slot] = 4;
+      }
     }
   }
 }
 
-__global__ void k_zero_counter(int32_t* counter) {
-  if (threadIdx.x == 0) counter[0] = 0;
+__global__ void k_zero_counter(int32_t* counter, int32_t n) {
+  if ((int32_t)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
 
 // Fixed-order finalisation of the slots flagged this launch.
@@ -184,7 +205,9 @@ static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {
       !b->chunk_flags_dev || !b->stats_dev || !b->counter_dev)
     return SDEMC_ERR_ARG;
   if (b->max_jobs < 1 || b->max_segments < 1 || b->partial_chunks < 0) return SDEMC_ERR_ARG;
-  if (b->class_set != 0 && b->class_set != 1) return SDEMC_ERR_ARG;
+  if (b->n_groups < 1 || b->n_groups > SDEMC_MAX_GROUPS) return SDEMC_ERR_ARG;
+  for (int g = 0; g < b->n_groups; ++g)
+    if (b->group_class[g] < 0 || b->group_class[g] > SDEMC_CLASS_GENERIC) return SDEMC_ERR_ARG;
   const int w = tune->waves_per_block;
   if (w != 1 && w != 2 && w != 4 && w != 8) return SDEMC_ERR_ARG;
   if (tune->blocks_per_cu < 1 || tune->blocks_per_cu > 64) return SDEMC_ERR_ARG;
@@ -203,24 +226,34 @@ static int cu_count_cached() {
   return cus;
 }
 
+template <int CLS>
+static void launch_group(const sdemc_batch_dev& b, int g, unsigned grid, int wpb, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch<CLS>, dim3(grid), dim3(WAVE * wpb), 0, s, (const KJob*)b.jobs_dev,
+                     b.segs_dev, b.seg_hdr_dev + 4 * g, b.partials_dev, b.chunk_flags_dev,
+                     b.counter_dev + g, b.partial_chunks, b.max_jobs, b.max_segments);
+}
+
 static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, b.counter_dev);
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, b.counter_dev, b.n_groups);
   const int wpb = tune.waves_per_block;
   const unsigned grid = (unsigned)(cu_count_cached() * tune.blocks_per_cu);
-  const KJob* jobs = (const KJob*)b.jobs_dev;
-  if (b.class_set == 0) {
-    hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.segs_dev,
-                       b.seg_hdr_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
-                       b.partial_chunks, b.max_jobs, b.max_segments);
-  } else {
-    hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(WAVE * wpb), 0, s, jobs, b.segs_dev,
-                       b.seg_hdr_dev, b.partials_dev, b.chunk_flags_dev, b.counter_dev,
-                       b.partial_chunks, b.max_jobs, b.max_segments);
+  for (int g = 0; g < b.n_groups; ++g) {
+    switch (b.group_class[g]) {
+      case 0: launch_group<0>(b, g, grid, wpb, s); break;
+      case 1: launch_group<1>(b, g, grid, wpb, s); break;
+      case 2: launch_group<2>(b, g, grid, wpb, s); break;
+      case 3: launch_group<3>(b, g, grid, wpb, s); break;
+      case 4: launch_group<4>(b, g, grid, wpb, s); break;
+      case 5: launch_group<5>(b, g, grid, wpb, s); break;
+      case 6: launch_group<6>(b, g, grid, wpb, s); break;
+      case 7: launch_group<7>(b, g, grid, wpb, s); break;
+      default: launch_group<SDEMC_CLASS_GENERIC>(b, g, grid, wpb, s); break;
+    }
   }
-  hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s, jobs,
-                     b.finalize_dev, b.partials_dev, b.chunk_flags_dev, b.partial_chunks,
-                     b.stats_dev);
+  hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s,
+                     (const KJob*)b.jobs_dev, b.finalize_dev, b.partials_dev, b.chunk_flags_dev,
+                     b.partial_chunks, b.stats_dev);
   return SDEMC_OK;
 }
 
@@ -237,10 +270,14 @@ extern "C" {
 
 size_t sdemc_sizeof_prepared_job(void) { return sizeof(KJob); }
 
+int sdemc_job_class(const sdemc_job* job) {
+  if (validate_job(job) != SDEMC_OK) return -1;
+  return job_class_of(job->model.noise, job->model.dim, job->grid, job->coarse_factor > 0);
+}
+
 int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_base,
-                       void* prepared_out, int32_t* class_set) {
+                       void* prepared_out, int32_t* classes) {
   if (n < 0 || (n > 0 && (!jobs || !chunk_base || !prepared_out))) return SDEMC_ERR_ARG;
-  int32_t set = 0;
   for (int32_t i = 0; i < n; ++i) {
     const int st = validate_job(&jobs[i]);
     if (st != SDEMC_OK) return st;
@@ -249,13 +286,11 @@ int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_ba
     memset(&k, 0, sizeof(k));
     make_kjob(jobs[i], k);
     k.chunk_base = chunk_base[i];
-    const sdemc_model& m = jobs[i].model;
-    const bool low = (m.noise == SDEMC_NOISE_DIAG && m.dim == 1) ||
-                     (m.noise == SDEMC_NOISE_INDEP_COMMON && m.dim == 2);
-    if (!low) set = 1;
     memcpy((char*)prepared_out + (size_t)i * sizeof(KJob), &k, sizeof(KJob));
+    if (classes)
+      classes[i] = job_class_of(jobs[i].model.noise, jobs[i].model.dim, jobs[i].grid,
+                                jobs[i].coarse_factor > 0);
   }
-  if (class_set) *class_set = set;
   return SDEMC_OK;
 }
 

@@ -22,10 +22,16 @@
 
 #include "../../include/sdemc.h"
 
+// No implicit multiply-add fusion anywhere below: every fused operation is
+// written as __fmaf_rn, so a path's arithmetic is the same instruction
+// sequence in every kernel this code is inlined into (plain, fixed-increment
+// and batched launches give the same bits for the same job).
+#pragma clang fp contract(off)
+
 namespace sdemc {
 
 enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2 };
-enum { STREAM_BM = 0, STREAM_JUMP = 1, STREAM_TEST = 7 };
+enum { STREAM_BM = 0, STREAM_JUMP = 1, STREAM_POIS = 2, STREAM_MARK = 3 };
 
 constexpr int WAVE = 64;
 
@@ -84,6 +90,33 @@ __device__ __forceinline__ float fast_ln(float v) {
   return 0.693147180559945309f * __builtin_amdgcn_logf(v);
 }
 
+// ln u for u = (w | 1) 2^-32 in (0,1), in fp32 from correctly rounded
+// operations only (division, fma, multiply, add), so a CPU reproduces it bit
+// for bit: u = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s),
+// s = (m-1)/(m+1), by its odd series to s^9 (|s| < 0.1716, error < 1e-9).
+__device__ __forceinline__ float ln_unit(uint32_t w) {
+#pragma clang fp contract(off)
+  const uint32_t v = w | 1u;
+  const int z = __clz((int)v);
+  const uint32_t top = v << z;
+  float m = (float)(top >> 8) * 0x1p-23f;
+  int e = -(z + 1);
+  const bool hi = m > 1.41421356f;
+  m = hi ? m * 0.5f : m;
+  e = hi ? e + 1 : e;
+  const float sn = m - 1.0f;
+  const float sd = m + 1.0f;
+  const float sq = __fdiv_rn(sn, sd);
+  const float s2 = sq * sq;
+  float p = __fmaf_rn(s2, 0.222222222f, 0.285714286f);
+  p = __fmaf_rn(s2, p, 0.4f);
+  p = __fmaf_rn(s2, p, 0.666666667f);
+  p = __fmaf_rn(s2, p, 2.0f);
+  const float lnm = sq * p;
+  const float et = (float)e * 0.693147181f;
+  return lnm + et;
+}
+
 // Normal number i of one path (i = step * K + k): word (i mod 4) of Philox
 // block i / 4 on the Brownian stream, paired (0,1) and (2,3) by Box-Muller.
 // The last block is cached, so sequential reads cost one Philox per 4 normals.
@@ -124,7 +157,8 @@ struct KModel {
   float jm, js;  // lognormal: mean and sd of the log-jump
   float lv[11];  // Levy inverse-CDF constants, see levy_icdf
   int32_t exact, law, dim, noise;
-  double rate, inv_rate;
+  double rate;
+  float inv_rate;  // 1 / lambda (fp32 jump clock)
 };
 
 struct KPayoff {
@@ -137,7 +171,7 @@ struct KJob {
   KModel m;
   KPayoff pay;
   double T;
-  double n_over_T;  // steps / T (fp64): jump arrival -> step index
+  float n_over_T;   // fp32(steps / T): jump arrival time -> step index
   float h;          // fp32(T / steps): fine step (reference: torch.tensor(T / N))
   float sqh;        // sqrt(h)
   float hc;         // coupled uniform grid: fp32(T * M / steps)
@@ -147,7 +181,33 @@ struct KJob {
   int64_t chunk_base;  // first chunk partial of this job (batched launches)
   uint32_t magic;      // KJOB_MAGIC once prepared on the host
   int32_t class_key;   // batched-kernel dispatch key, see class_key_of
+  // per-step Poisson counts (uniform grid, Levy marks): step n has at least
+  // k+1 jumps iff its uniform word w < pois_T[k]; the first mark is the
+  // offset of w within its count's interval (see poisson_count)
+  int32_t pois_mode;
+  uint32_t pois_T[16];
+  double pois_inv;
 };
+
+// Thresholds of the per-step jump count: pois_T[k] = round(P(N >= k+1) 2^32)
+// for N ~ Poisson(mu), mu = rate * T / steps; the tail sums run from the top
+// so small tails keep full relative precision.  Returns whether counts above
+// 16 are negligible (P(N > 16) < 2^-33).
+inline bool poisson_thresholds(double mu, uint32_t* T16, double* inv) {
+#pragma clang fp contract(off)
+  double p[64];
+  p[0] = exp(-mu);
+  for (int i = 1; i < 64; ++i) p[i] = p[i - 1] * mu / (double)i;
+  double tail[65];
+  tail[64] = 0.0;
+  for (int i = 63; i >= 0; --i) tail[i] = tail[i + 1] + p[i];
+  for (int k = 1; k <= 16; ++k) {
+    const double v = tail[k] * 4294967296.0;
+    T16[k - 1] = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(v + 0.5);
+  }
+  *inv = T16[0] ? 1.0 / (double)T16[0] : 0.0;
+  return tail[17] * 4294967296.0 < 0.5;
+}
 
 constexpr uint32_t KJOB_MAGIC = 0x5DE3C0DEu;
 
@@ -195,7 +255,7 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
     f.lv[10] = (float)m.jp[10];                             // -1/alpha
   }
   f.rate = m.rate;
-  f.inv_rate = m.rate > 0.0 ? 1.0 / m.rate : 0.0;
+  f.inv_rate = m.rate > 0.0 ? (float)(1.0 / m.rate) : 0.0f;
   k.pay.kind = j.payoff.kind;
   k.pay.log = j.payoff.log;
   k.pay.strike = (float)j.payoff.strike;
@@ -206,7 +266,7 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.grid = j.grid;
   k.M = j.coarse_factor;
   k.max_steps = j.max_steps;
-  k.n_over_T = (double)j.steps / j.T;
+  k.n_over_T = (float)((double)j.steps / j.T);
   k.h = (float)(j.T / (double)j.steps);
   k.sqh = sqrtf(k.h);
   if (j.coarse_factor > 0) {
@@ -222,6 +282,13 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.chunk_base = 0;
   k.magic = KJOB_MAGIC;
   k.class_key = class_key_of(m.noise, m.dim, j.grid, j.coarse_factor > 0);
+  k.pois_mode = 0;
+  for (int q = 0; q < 16; ++q) k.pois_T[q] = 0;
+  k.pois_inv = 0.0;
+  if (m.jump_law == SDEMC_JUMP_LEVY && j.grid == SDEMC_GRID_UNIFORM && m.rate > 0.0) {
+    const double mu = m.rate * j.T / (double)j.steps;
+    k.pois_mode = poisson_thresholds(mu, k.pois_T, &k.pois_inv) ? 1 : 0;
+  }
 }
 
 // Inverse CDF of the Levy jump law of levy.py:22-30 on y in (0,1):
@@ -234,13 +301,13 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
 __device__ __forceinline__ float levy_icdf(const KModel& f, float y) {
   float x;
   if (y <= f.lv[0]) {
-    x = fast_ln(f.lv[3] * y) * f.lv[4] - 1.0f;
+    x = __fmaf_rn(fast_ln(f.lv[3] * y), f.lv[4], -1.0f);
   } else if (y < f.lv[1]) {
-    x = -fast_pow(f.lv[5] * y + f.lv[6], f.lv[10]);
+    x = -fast_pow(__fmaf_rn(f.lv[5], y, f.lv[6]), f.lv[10]);
   } else if (y < f.lv[2]) {
-    x = fast_pow(f.lv[7] * y + f.lv[8], f.lv[10]);
+    x = fast_pow(__fmaf_rn(f.lv[7], y, f.lv[8]), f.lv[10]);
   } else {
-    x = 1.0f - f.lv[4] * fast_ln(f.lv[9] * (1.0f - y));
+    x = __fmaf_rn(-f.lv[4], fast_ln(f.lv[9] * (1.0f - y)), 1.0f);
   }
   return x;
 }
@@ -256,7 +323,7 @@ __device__ __forceinline__ float jump_mark(const KModel& f, uint32_t m1, uint32_
   const float a = u01_half(m2);
   const float z = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u)) *
                   __builtin_amdgcn_cosf(a);
-  return fast_exp(f.jm + f.js * z) - 1.0f;
+  return fast_exp(__fmaf_rn(f.js, z, f.jm)) - 1.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -320,11 +387,15 @@ __device__ __forceinline__ void euler(const KModel& f, float (&x)[D], float dt,
                                       const float (&dW)[D], const float (&dB)[D]) {
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const float a = f.a0[i] + f.a1[i] * x[i];
-    const float b = f.b0[i] + f.b1[i] * x[i];
-    float diff = b * dW[i];
-    if (FAM != FAM_MERTON) diff += (f.c0[i] + f.c1[i] * x[i]) * dB[i];
-    x[i] = x[i] + a * dt + diff;
+    const float a = __fmaf_rn(f.a1[i], x[i], f.a0[i]);
+    const float b = __fmaf_rn(f.b1[i], x[i], f.b0[i]);
+    const float t = __fmaf_rn(a, dt, x[i]);
+    if (FAM == FAM_MERTON) {
+      x[i] = __fmaf_rn(b, dW[i], t);
+    } else {
+      const float c = __fmaf_rn(f.c1[i], x[i], f.c0[i]);
+      x[i] = t + __fmaf_rn(c, dB[i], b * dW[i]);
+    }
   }
 }
 
@@ -333,7 +404,7 @@ template <int D>
 __device__ __forceinline__ void add_jump(const KModel& f, float (&x)[D], const float (&y)[D],
                                          float J) {
 #pragma unroll
-  for (int i = 0; i < D; ++i) x[i] = x[i] + (f.j0[i] + f.j1[i] * y[i]) * J;
+  for (int i = 0; i < D; ++i) x[i] = __fmaf_rn(__fmaf_rn(f.j1[i], y[i], f.j0[i]), J, x[i]);
 }
 
 // Number of normals drawn per step.
@@ -355,9 +426,9 @@ __device__ __forceinline__ void correlate(const KModel& f, const float* z, float
   } else {
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      float acc = 0.0f;
+      float acc = f.L[i][0] * zs[0];
 #pragma unroll
-      for (int j = 0; j <= i; ++j) acc += f.L[i][j] * zs[j];
+      for (int j = 1; j <= i; ++j) acc = __fmaf_rn(f.L[i][j], zs[j], acc);
       dW[i] = acc;
     }
   }
@@ -371,9 +442,9 @@ __device__ __forceinline__ void correlate(const KModel& f, const float* z, float
     for (int j = 0; j < D; ++j) zb[j] = z[D + j] * sq;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      float acc = 0.0f;
+      float acc = f.L[i][0] * zb[0];
 #pragma unroll
-      for (int j = 0; j <= i; ++j) acc += f.L[i][j] * zb[j];
+      for (int j = 1; j <= i; ++j) acc = __fmaf_rn(f.L[i][j], zb[j], acc);
       dB[i] = acc;
     }
   } else {
@@ -388,34 +459,89 @@ __device__ __forceinline__ void correlate(const KModel& f, const float* z, float
 // words y, z its mark.  Arrival times accumulate in fp64.
 // ---------------------------------------------------------------------------
 struct JumpClock {
-  double tau;   // arrival time of jump j
+  float tau;    // arrival time of jump j (fp32, accumulated gap by gap)
   uint32_t j;   // index of the pending jump
   uint32_t m1, m2;
   __device__ __forceinline__ void draw(uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi,
-                                       double inv_rate) {
+                                       float inv_rate) {
+#pragma clang fp contract(off)
     const U4 w = philox4x32_10(j, STREAM_JUMP, plo, phi, k0, k1);
-    const double u = ((double)w.x + 0.5) * 0x1p-32;
-    tau += -log(u) * inv_rate;
+    const float gap = -ln_unit(w.x) * inv_rate;
+    tau = tau + gap;
     m1 = w.y;
     m2 = w.z;
   }
   __device__ __forceinline__ void init(uint32_t k0, uint32_t k1, uint32_t plo, uint32_t phi,
-                                       double rate, double inv_rate) {
+                                       double rate, float inv_rate) {
     j = 0;
-    tau = 0.0;
+    tau = 0.0f;
     if (rate > 0.0) {
       draw(k0, k1, plo, phi, inv_rate);
     } else {
-      tau = __builtin_inf();
+      tau = __builtin_inff();
       m1 = m2 = 0;
     }
   }
 };
 
-// Step index (on a grid of n steps over [0,T]) in which time tau falls.
-__device__ __forceinline__ int32_t step_of(double tau, double n_over_T, int32_t n) {
-  const double q = tau * n_over_T;
-  return q < (double)n ? (int32_t)q : 0x7fffffff;
+// Step index (grid of n steps over [0,T]) in which time tau falls.
+__device__ __forceinline__ int32_t step_of(float tau, float n_over_T, int32_t n) {
+#pragma clang fp contract(off)
+  const float q = tau * n_over_T;
+  return q < (float)n ? (int32_t)q : 0x7fffffff;
+}
+
+// Count of uniform-grid step n in per-step Poisson mode: the count is c iff
+// pois_T[c] <= w < pois_T[c-1] (pois_T[16] = 0), and given c the offset of
+// w in that interval is uniform and independent of c, so it yields the
+// first mark's uniform: y = (w - lo + 1/2) / (hi - lo), fp64, one rounding.
+__device__ __forceinline__ int poisson_count(const KJob& j, uint32_t w, float& y) {
+  int c = 1;
+  while (c < 16 && w < j.pois_T[c]) ++c;
+  const uint32_t lo = c < 16 ? j.pois_T[c] : 0u;
+  const uint32_t hi = j.pois_T[c - 1];
+  const double r = ((double)(w - lo) + 0.5) / (double)(hi - lo);
+  y = fminf((float)r, 0x1.fffffep-1f);
+  return c;
+}
+
+// Uniform of mark k >= 1 of step n: word (k-1) mod 4 of block n of the mark
+// stream (stream index 3 + 16 floor((k-1)/4)).
+__device__ __forceinline__ float poisson_mark_uniform(uint32_t k0, uint32_t k1, uint32_t plo,
+                                                      uint32_t phi, int32_t n, int k) {
+  const uint32_t q = (uint32_t)(k - 1);
+  const U4 mw = philox4x32_10((uint32_t)n, STREAM_MARK + 16u * (q >> 2), plo, phi, k0, k1);
+  const uint32_t sel = q & 3u;
+  return u01_open(sel == 0 ? mw.x : (sel == 1 ? mw.y : (sel == 2 ? mw.z : mw.w)));
+}
+
+// Jumps of uniform-grid step n in per-step Poisson mode, given the step's
+// count word w (w < pois_T[0]): marks are added with coefficients at xo
+// (pre-step state) or at the running state (exact jumps).  Returns the sum
+// of the marks.
+template <int D>
+__device__ __forceinline__ float poisson_jumps(const KJob& j, uint32_t k0, uint32_t k1,
+                                               uint32_t plo, uint32_t phi, uint32_t w,
+                                               int32_t n, float (&x)[D],
+                                               const float (&xo)[D]) {
+  const KModel& f = j.m;
+  float y;
+  const int c = poisson_count(j, w, y);
+  float jsum = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    if (k > 0) y = poisson_mark_uniform(k0, k1, plo, phi, n, k);
+    const float J = levy_icdf(f, y);
+    if (f.exact) {
+      float yy[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) yy[i] = x[i];
+      add_jump(f, x, yy, J);
+    } else {
+      add_jump(f, x, xo, J);
+    }
+    jsum += J;
+  }
+  return jsum;
 }
 
 // ---------------------------------------------------------------------------
@@ -506,9 +632,13 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
   const float h = j.h;
   const float sq = j.sqh;
   const uint32_t plo = (uint32_t)c.path, phi = (uint32_t)(c.path >> 32);
+  const bool pois = j.pois_mode != 0;
   JumpClock jc;
-  jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
-  int32_t kj = step_of(jc.tau, j.n_over_T, N);
+  int32_t kj = 0x7fffffff;
+  if (!pois) {
+    jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
+    kj = step_of(jc.tau, j.n_over_T, N);
+  }
 #pragma unroll
   for (int i = 0; i < D; ++i) x[i] = f.x0[i];
   if (STORE && lane_valid) {
@@ -524,6 +654,8 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
       box_muller(w.x, w.y, z[4 * b + 0], z[4 * b + 1]);
       box_muller(w.z, w.w, z[4 * b + 2], z[4 * b + 3]);
     }
+    U4 pw = {0, 0, 0, 0};
+    if (pois) pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int32_t n = n0 + s;
@@ -534,7 +666,10 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
         for (int i = 0; i < D; ++i) xo[i] = x[i];
         euler<D, FAM>(f, x, h, dW, dB);
         float jsum = 0.0f;
-        if (n == kj) {
+        if (pois) {
+          const uint32_t w = s == 0 ? pw.x : (s == 1 ? pw.y : (s == 2 ? pw.z : pw.w));
+          if (w < j.pois_T[0]) jsum = poisson_jumps<D>(j, c.k0, c.k1, plo, phi, w, n, x, xo);
+        } else if (n == kj) {
           do {
             const float J = jump_mark(f, jc.m1, jc.m2);
             if (f.exact) {
@@ -676,7 +811,7 @@ __device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], fl
         xo[i] = xf[i];
       }
       euler<D, FAM>(f, xf, j.h, dW, dB);
-      while (src.jump_pending(n)) {
+      while (src.jump_pending(j, n)) {
         const float J = src.take_jump(j);
         if (f.exact) {
           float y[D];
@@ -685,9 +820,9 @@ __device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], fl
           add_jump(f, xf, y, J);
 #pragma unroll
           for (int i = 0; i < D; ++i) {
-            const float g = 1.0f + f.j1[i] * J;
+            const float g = __fmaf_rn(f.j1[i], J, 1.0f);
             A[i] = A[i] * g;
-            Bc[i] = Bc[i] * g + f.j0[i] * J;
+            Bc[i] = __fmaf_rn(Bc[i], g, f.j0[i] * J);
           }
         } else {
           add_jump(f, xf, xo, J);
@@ -701,38 +836,77 @@ __device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], fl
     euler<D, FAM>(f, xc, j.hc, sW, sB);
     if (f.exact) {
 #pragma unroll
-      for (int i = 0; i < D; ++i) xc[i] = A[i] * xc[i] + Bc[i];
+      for (int i = 0; i < D; ++i) xc[i] = __fmaf_rn(A[i], xc[i], Bc[i]);
     } else if (Jsum != 0.0f) {
       add_jump(f, xc, xco, Jsum);
     }
   }
 }
 
-// RNG source for the uniform coupled path.
+// RNG source for the uniform coupled path: fine-step normals, and jumps
+// either from the arrival clock or from per-step Poisson counts.
 template <int D, int FAM>
 struct UniRngSrc {
   NormalStream ns;
   JumpClock jc;
   uint32_t plo, phi, k0, k1;
   int32_t kj;
+  // per-step Poisson mode: cached count block, current word and mark index
+  uint32_t pblk, pw0, pw1, pw2, pw3, w;
+  int32_t cur_n, k, cnt;
+  float y0;
   __device__ __forceinline__ void init(const KJob& j, const PathCtx& c) {
     k0 = c.k0;
     k1 = c.k1;
     plo = (uint32_t)c.path;
     phi = (uint32_t)(c.path >> 32);
     ns.init(k0, k1, c.path);
-    jc.init(k0, k1, plo, phi, j.m.rate, j.m.inv_rate);
-    kj = step_of(jc.tau, j.n_over_T, j.steps);
+    kj = 0x7fffffff;
+    pblk = 0xffffffffu;
+    cur_n = -1;
+    k = 0;
+    cnt = 0;
+    y0 = 0.5f;
+    w = 0xffffffffu;
+    if (!j.pois_mode) {
+      jc.init(k0, k1, plo, phi, j.m.rate, j.m.inv_rate);
+      kj = step_of(jc.tau, j.n_over_T, j.steps);
+    }
   }
   __device__ __forceinline__ void incr(const KJob& j, int32_t n, float (&dW)[D], float (&dB)[D]) {
     constexpr int K = NK<D, FAM>::K;
     float z[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) z[k] = ns.get((uint32_t)n * K + k);
+    for (int q = 0; q < K; ++q) z[q] = ns.get((uint32_t)n * K + q);
     correlate<D, FAM>(j.m, z, j.sqh, dW, dB);
+    if (j.pois_mode) {
+      const uint32_t b = (uint32_t)n >> 2;
+      if (b != pblk) {
+        pblk = b;
+        const U4 pw = philox4x32_10(b, STREAM_POIS, plo, phi, k0, k1);
+        pw0 = pw.x;
+        pw1 = pw.y;
+        pw2 = pw.z;
+        pw3 = pw.w;
+      }
+      const uint32_t sel = (uint32_t)n & 3u;
+      w = sel == 0 ? pw0 : (sel == 1 ? pw1 : (sel == 2 ? pw2 : pw3));
+      cur_n = n;
+      k = 0;
+      cnt = 0;
+      if (w < j.pois_T[0]) cnt = poisson_count(j, w, y0);
+    }
   }
-  __device__ __forceinline__ bool jump_pending(int32_t n) const { return kj == n; }
+  __device__ __forceinline__ bool jump_pending(const KJob& j, int32_t n) const {
+    if (j.pois_mode) return n == cur_n && k < cnt;
+    return kj == n;
+  }
   __device__ __forceinline__ float take_jump(const KJob& j) {
+    if (j.pois_mode) {
+      const float y = k == 0 ? y0 : poisson_mark_uniform(k0, k1, plo, phi, cur_n, k);
+      ++k;
+      return levy_icdf(j.m, y);
+    }
     const float J = jump_mark(j.m, jc.m1, jc.m2);
     ++jc.j;
     jc.draw(k0, k1, plo, phi, j.m.inv_rate);
@@ -763,7 +937,9 @@ struct UniFixedSrc {
     cur_n = n;
     left = in->counts_dev ? (int)in->counts_dev[(int64_t)n * P + p] : 0;
   }
-  __device__ __forceinline__ bool jump_pending(int32_t n) const { return n == cur_n && left > 0; }
+  __device__ __forceinline__ bool jump_pending(const KJob&, int32_t n) const {
+    return n == cur_n && left > 0;
+  }
   __device__ __forceinline__ float take_jump(const KJob&) {
     --left;
     const bool ok = off >= 0 && off < in->n_marks;
@@ -804,7 +980,7 @@ struct AdaptRngSrc {
     constexpr int K = NK<D, FAM>::K;
     return ns.get(row * K + k);
   }
-  __device__ __forceinline__ float next_time() const { return (float)jc.tau; }
+  __device__ __forceinline__ float next_time() const { return jc.tau; }
   __device__ __forceinline__ float mark(const KModel& f) const {
     return jump_mark(f, jc.m1, jc.m2);
   }

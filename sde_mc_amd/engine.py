@@ -133,6 +133,8 @@ def check_flags(flags, where):
         raise L.SdemcError(L.ERR_OVERFLOW, where)
     if f & 2:
         raise L.SdemcError(L.ERR_ARG, where + ' (jump CSR read outside jump_size)')
+    if f & 4:
+        raise L.SdemcError(L.ERR_UNSUPPORTED, where + ' (job outside the launch class set)')
 
 
 def stats_to_host(stats, where='simulate'):

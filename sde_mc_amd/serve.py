@@ -116,7 +116,7 @@ class PricingServer:
         return time.perf_counter() - self.t0
 
     def _admit(self):
-        staged, cls = [], 0
+        staged = []
         while self.queue and self.free_slots:
             req = self.queue[0]
             nc = chunks_of(req.paths)
@@ -140,15 +140,17 @@ class PricingServer:
             staged.append(a)
         self.peak_used = max(self.peak_used, self.pool.used)
         if staged:
-            cls = self.runner.prepare([a.job for a in staged], [a.base for a in staged],
-                                      [a.slot for a in staged])
+            self.runner.prepare([a.job for a in staged], [a.base for a in staged],
+                                [a.slot for a in staged])
             self.runner.upload_slots([a.slot for a in staged])
-        return cls
 
     # -- rounds ------------------------------------------------------------------
     def _plan(self):
-        """Fair share of the round's cost budget over the admitted requests with
-        chunks left; a request finishing its chunks this round is finalised."""
+        """Split the round's cost budget over the admitted requests with chunks
+        left, oldest first: each takes at most half of what is left (at least
+        one chunk), so the oldest request keeps most of the device while every
+        newcomer still gets work into the round.  A request whose last chunk
+        is in the round is finalised in it."""
         todo = [a for a in self.active.values() if a.next < a.nc]
         if not todo:
             return None
@@ -156,15 +158,18 @@ class PricingServer:
         budget = self.cost_rate * self.round_s
         take = {a.slot: 0 for a in todo}
         left = budget
-        for _ in range(3):
-            live = [a for a in todo if a.next + take[a.slot] < a.nc]
-            if not live or left <= 0:
-                break
-            share = left / len(live)
-            for a in live:
-                k = min(a.nc - a.next - take[a.slot], max(1, int(share / a.cost)))
+        for _ in range(2):
+            for a in todo:
+                rem = a.nc - a.next - take[a.slot]
+                if rem <= 0:
+                    continue
+                if left <= 0 and take[a.slot] > 0:
+                    continue
+                k = min(rem, max(1, int(max(0.5 * left, 0.0) / a.cost)))
                 take[a.slot] += k
                 left -= k * a.cost
+            if left <= 0:
+                break
         slots, firsts, counts, costs, final = [], [], [], [], []
         for a in todo:
             k = take[a.slot]
@@ -182,22 +187,16 @@ class PricingServer:
     def _launch(self, plan):
         slots, firsts, counts, costs, final = plan
         buf = next(b for b in self.runner.bufs if b not in [f[0] for f in self.in_flight])
-        segs, ntiles = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks)
-        self.runner.stage_round(buf, segs, ntiles, final)
-        cls = 0 if all(self._class0(self.active[s].job) for s in slots) else 1
+        rp = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks,
+                           self.runner.slot_class[np.asarray(slots, dtype=np.int64)])
+        self.runner.stage_round(buf, rp, final)
         start = torch.cuda.Event(enable_timing=True)
         start.record()
-        self.runner.launch(buf, cls)
+        self.runner.launch(buf, rp.groups)
         self.runner.read_stats(buf, final)
         cost = float(np.dot(counts, costs))
         self.in_flight.append((buf, final, start, cost))
         self.rounds += 1
-
-    @staticmethod
-    def _class0(job):
-        m = job.model
-        return (m.noise == L.NOISE_DIAG and m.dim == 1) or (m.noise == L.NOISE_INDEP_COMMON
-                                                            and m.dim == 2)
 
     def _complete(self, block):
         """Retire finished rounds (the oldest first); block on it if asked."""

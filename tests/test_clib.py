@@ -58,13 +58,15 @@ def _prepare(job):
     return st, cs.value
 
 
-def test_prepare_accepts_workloads_and_picks_class_set():
+def test_prepare_accepts_workloads_and_reports_classes():
     st, cs = _prepare(J.job_for(J.merton_sde(), 1.0, 500, 1 << 22))
     assert st == L.OK and cs == 0
     st, cs = _prepare(J.job_for(J.levy_rainbow_sde(), 1.0, 200, 1 << 22, grid='adapted'))
-    assert st == L.OK and cs == 0
-    st, cs = _prepare(J.job_for(J.merton_sde(dim=3), 1.0, 50, 100))
+    assert st == L.OK and cs == 6
+    st, cs = _prepare(J.job_for(J.merton_sde(), 1.0, 200, 1 << 10, coarse_factor=2))
     assert st == L.OK and cs == 1
+    st, cs = _prepare(J.job_for(J.merton_sde(dim=3), 1.0, 50, 100))
+    assert st == L.OK and cs == L.CLASS_GENERIC
 
 
 @pytest.mark.parametrize("mutate,status", [

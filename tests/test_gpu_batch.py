@@ -66,10 +66,41 @@ def test_graph_replays_are_consistent():
     assert np.array_equal(r2.run(jobs), first)
 
 
-def test_all_class_set():
-    # 3-D and 4-D models take the full class set of the persistent kernel
+def test_generic_class_with_dedicated_classes():
+    # a 3-D model runs in the generic kernel, next to a dedicated 1-D group
     corr = torch.tensor([[1.0, 0.2, 0.1], [0.2, 1.0, 0.3], [0.1, 0.3, 1.0]])
     jobs = [J.job_for(J.merton_sde(dim=3, corr=corr), 1.0, 32, 3000, option=sm.Rainbow(1.0), seed=7),
             J.job_for(J.merton_sde(), 1.0, 40, 5000, option=sm.EuroCall(1.0), seed=8)]
     plain = np.array([_plain(j) for j in jobs])
     assert np.array_equal(BatchRunner(max_jobs=8).run(jobs), plain)
+
+
+def test_class_mismatch_is_flagged_not_silent():
+    # a coupled job planned into the single-level kernel's group is refused by
+    # the kernel (NaN partials, flag 4), which the host reports
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd.batch import plan_segments
+    job = J.job_for(J.merton_sde(), 1.0, 32, 500, coarse_factor=2, option=sm.EuroCall(1.0))
+    r = BatchRunner(max_jobs=4)
+    assert list(r.prepare([job], [0], [0])) == [1]
+    r.upload_slots([0])
+    plan = plan_segments([0], [0], [8], [1.0], 1, [0])
+    buf = r.bufs[0]
+    r.stage_round(buf, plan, [0])
+    r.launch(buf, plan.groups)
+    r.read_stats(buf, [0])
+    buf.done.synchronize()
+    st = buf.h_stats[:4].numpy()
+    assert int(st[3]) & 4 and np.isnan(st[0])
+    with pytest.raises(L.SdemcError):
+        E.check_flags(st[3], 'job')
+
+
+def test_autotune_picks_a_candidate_and_it_prices_the_same():
+    from sde_mc_amd.autotune import autotune
+    jobs = _mix(6, seed=11)
+    cands = [BatchTune(2, 4, 1), BatchTune(4, 8, 2)]
+    tune, table = autotune(jobs, candidates=cands, use_cache=False)
+    assert tune.key() in [c.key() for c in cands] and len(table) == 2
+    plain = np.array([_plain(j) for j in jobs])
+    assert np.array_equal(BatchRunner(max_jobs=16, tune=tune).run(jobs), plain)

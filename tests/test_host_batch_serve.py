@@ -13,31 +13,62 @@ def test_chunks_of():
     assert [chunks_of(n) for n in (1, 63, 64, 65, 1 << 22)] == [1, 1, 1, 2, 1 << 16]
 
 
+def _expand(plan):
+    """(slot, chunk) -> times covered, walking each group's tiles the way the
+    persistent kernel does (tile t -> the last segment with tile_begin <= t)."""
+    seen = {}
+    for g in range(len(plan.groups)):
+        s0, ns, tiles, _ = plan.hdr[g]
+        segs = plan.segs[s0:s0 + ns]
+        for t in range(tiles):
+            s = segs[np.searchsorted(segs[:, 4], t, side='right') - 1]
+            c0 = s[1] + (t - s[4]) * s[3]
+            for c in range(c0, min(c0 + s[3], s[1] + s[2])):
+                seen[(s[0], c)] = seen.get((s[0], c), 0) + 1
+    return seen
+
+
 def test_segments_cover_every_chunk_once_costliest_first():
     rng = np.random.default_rng(0)
     n = 50
     counts = rng.integers(0, 300, n)
     firsts = rng.integers(0, 1000, n)
     costs = rng.uniform(1, 100, n)
-    for per in (1, 2, 4, 7):
-        segs, total = plan_segments(np.arange(n), firsts, counts, costs, per)
-        assert len(segs) == int((counts > 0).sum())
-        # consecutive tile ranges from 0
-        nt = (segs[:, 2] + segs[:, 3] - 1) // segs[:, 3]
-        assert segs[0, 4] == 0 and np.array_equal(segs[1:, 4], np.cumsum(nt)[:-1])
-        assert total == nt.sum()
-        # expand tiles -> chunks: exactly the requested ranges
-        seen = {}
-        for s in segs:
-            for t in range((s[2] + s[3] - 1) // s[3]):
-                c0 = s[1] + t * s[3]
-                for c in range(c0, min(c0 + s[3], s[1] + s[2])):
-                    seen[(s[0], c)] = seen.get((s[0], c), 0) + 1
-        want = {(i, firsts[i] + c): 1 for i in range(n) for c in range(counts[i])}
-        assert seen == want
-        # costliest first
-        order_costs = costs[segs[:, 0]]
-        assert np.all(np.diff(order_costs) <= 0)
+    for classes in (None, rng.integers(0, 9, n), np.full(n, 3)):
+        for per in (1, 2, 4, 7):
+            plan = plan_segments(np.arange(n), firsts, counts, costs, per, classes)
+            segs = plan.segs
+            assert len(segs) == int((counts > 0).sum())
+            assert len(plan.groups) <= L.MAX_GROUPS
+            assert plan.hdr[:len(plan.groups), 1].sum() == len(segs)
+            nt = (segs[:, 2] + segs[:, 3] - 1) // segs[:, 3]
+            assert plan.tiles == nt.sum() == plan.hdr[:, 2].sum()
+            for g, c in enumerate(plan.groups):
+                s0, ns, tiles, _ = plan.hdr[g]
+                gs = segs[s0:s0 + ns]
+                # consecutive tile ranges from 0 within the group
+                gnt = (gs[:, 2] + gs[:, 3] - 1) // gs[:, 3]
+                assert gs[0, 4] == 0 and np.array_equal(gs[1:, 4], np.cumsum(gnt)[:-1])
+                assert tiles == gnt.sum()
+                # costliest first within the group
+                assert np.all(np.diff(costs[gs[:, 0]]) <= 0)
+                if classes is not None and c != L.CLASS_GENERIC:
+                    assert np.all(classes[gs[:, 0]] == c)
+            # expand tiles -> chunks: exactly the requested ranges
+            want = {(i, firsts[i] + c): 1 for i in range(n) for c in range(counts[i])}
+            assert _expand(plan) == want
+
+
+def test_nine_classes_fold_into_eight_groups():
+    n = 9
+    plan = plan_segments(np.arange(n), np.zeros(n), np.full(n, 4), np.arange(1.0, n + 1), 1,
+                         np.arange(n))
+    assert len(plan.groups) == L.MAX_GROUPS and L.CLASS_GENERIC in plan.groups
+    # the cheapest dedicated class (0, cost 1) joins the generic group
+    assert 0 not in plan.groups
+    g = plan.groups.index(L.CLASS_GENERIC)
+    s0, ns, _, _ = plan.hdr[g]
+    assert sorted(plan.segs[s0:s0 + ns, 0]) == [0, 8]
 
 
 def test_job_cost_orders_levy_above_merton():
@@ -91,6 +122,21 @@ def test_request_jobs_validate_on_host():
         job = W.request_job(r)
         buf = ctypes.create_string_buffer(lib.sdemc_sizeof_prepared_job())
         base = (ctypes.c_int64 * 1)(0)
-        cs = ctypes.c_int32()
+        cs = ctypes.c_int32(-1)
         assert lib.sdemc_prepare_jobs(ctypes.byref(job), 1, base, buf, ctypes.byref(cs)) == L.OK
-        assert cs.value == 0
+        assert cs.value in (0, 4)  # 1-D Merton or 2-D Levy, single level, uniform grid
+
+
+def test_job_classes():
+    from sde_mc_amd.batch import job_class
+    m1 = J.job_for(J.merton_sde(), 1.0, 10, 100)
+    m1a = J.job_for(J.merton_sde(), 1.0, 10, 100, grid='adapted')
+    m1c = J.job_for(J.merton_sde(), 1.0, 10, 100, coarse_factor=2)
+    l2 = J.job_for(J.levy_rainbow_sde(), 1.0, 10, 100)
+    l2ac = J.job_for(J.levy_rainbow_sde(), 1.0, 10, 100, grid='adapted', coarse_factor=2)
+    m3 = J.job_for(J.merton_sde(dim=3), 1.0, 10, 100)
+    assert [job_class(j) for j in (m1, m1a, m1c, l2, l2ac, m3)] == [0, 2, 1, 4, 7, 8]
+    bad = J.job_for(J.merton_sde(), 1.0, 10, 100)
+    bad.steps = 0
+    with pytest.raises(L.SdemcError):
+        job_class(bad)

@@ -236,3 +236,51 @@ def test_oracle_mlmc_telescopes():
     assert abs(d.mean() - diff) < 4 * se
     # the coupled correction has far smaller variance than either level
     assert d.var() < 0.05 * pf.var()
+
+
+# ---- jump draws ------------------------------------------------------------
+def test_ln_unit_accuracy():
+    rng = np.random.default_rng(3)
+    for w in list(rng.integers(0, 2 ** 32, 20000)) + [0, 1, 2 ** 31, 2 ** 32 - 1]:
+        exact = math.log((int(w) | 1) * 2.0 ** -32)
+        # 24 significant bits of w are used: |error| <= 2^-24 relative in u
+        assert abs(oracle.ln_unit(w) - exact) <= 2e-7 * abs(exact) + 7e-8
+
+
+def test_poisson_thresholds_match_scipy():
+    for mu in (1e-4, 0.004, 0.125, 0.5, 1.0, 3.0):
+        T, inv, ok = oracle.poisson_thresholds(mu)
+        # counts above 16 must be negligible for the per-step mode to apply
+        assert ok == (stats.poisson.sf(16, mu) * 2 ** 32 < 0.5)
+        for k in range(1, 17):
+            assert abs(T[k - 1] - stats.poisson.sf(k - 1, mu) * 2 ** 32) <= 0.51
+        assert inv == 1.0 / T[0]
+
+
+def test_per_step_counts_are_poisson_and_marks_follow_the_levy_law():
+    # the Levy rainbow at 50 steps per year: lambda h = 24.97 / 50 ~ 0.5
+    sde = J.levy_rainbow_sde()
+    job = J.job_for(sde, 1.0, 50, 1, seed=77)
+    mu = job.model.rate / 50
+    counts, firsts, firsts_multi = [], [], []
+    for p in range(4000):
+        for n in range(0, 50, 5):
+            c, fm = oracle.step_jumps(job, p, n)
+            counts.append(c)
+            if c:
+                firsts.append(fm)
+            if c >= 2:
+                firsts_multi.append(fm)
+    counts = np.array(counts)
+    kmax = 5
+    obs = np.array([(counts == k).sum() for k in range(kmax)] + [(counts >= kmax).sum()])
+    pmf = np.array([stats.poisson.pmf(k, mu) for k in range(kmax)] + [stats.poisson.sf(kmax - 1, mu)])
+    chi2 = ((obs - pmf * counts.size) ** 2 / (pmf * counts.size)).sum()
+    assert chi2 < stats.chi2.ppf(0.999, kmax)
+    # the residual mark: the truncated |z|^-1.8 law on 0.05 < |z| < 1
+    lda = job.model.jp[5]
+    cdf = lambda z: np.array([_levy_cdf(v, 1.0, 1.0, math.inf, 0.8, 0.05, lda) for v in z])
+    assert stats.kstest(np.array(firsts), cdf).pvalue > 1e-3
+    # ... independently of the count: the first mark of multi-jump steps too
+    assert len(firsts_multi) > 500
+    assert stats.kstest(np.array(firsts_multi), cdf).pvalue > 1e-3
