@@ -62,6 +62,164 @@ def cpu_baseline(reqs, budget_s=12.0, cap_paths=1 << 12):
                       "loop op for op in torch fp32 (%.1f s)" % (n_req, cap_paths, t_total)}
 
 
+MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2, N(-0.1, 0.15^2), T=1
+
+
+def _time_events(fn, reps):
+    """Mean device time (ms) of fn() over reps, bracketed by events on the
+    current stream (the stream the library launches on)."""
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def fixed_increment_leg(dev, reps, P=1 << 21, N=500, seed=0):
+    """1-D Merton call on the uniform grid fed from HBM (the parity mode at a
+    size well past the 256 MB last-level cache): dw [N][1][P] fp32 N(0, h),
+    counts [N][P] uint8 Poisson(lambda h), log-jump marks exp(N(-0.1, 0.15^2))-1
+    in path-major CSR order.  Inputs are drawn on the device (synthetic)."""
+    import torch
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import workloads as W
+    from sde_mc_amd.options import EuroCall
+    T = 1.0
+    h = T / N
+    g = torch.Generator(device=dev).manual_seed(seed)
+    dw = torch.randn((N, 1, P), device=dev, generator=g).mul_(math.sqrt(h))
+    counts = torch.poisson(torch.full((N, P), 2.0 * h, device=dev), generator=g).to(torch.uint8)
+    per_path = counts.sum(dim=0, dtype=torch.int64)
+    off = torch.zeros(P + 1, dtype=torch.int64, device=dev)
+    off[1:] = torch.cumsum(per_path, 0)
+    n_marks = int(off[-1].item())
+    js = torch.randn(max(n_marks, 1), device=dev, generator=g).mul_(0.15).add_(-0.1).exp_().sub_(1.0)
+    fin = L.FixedIn()
+    fin.dw_dev, fin.counts_dev, fin.jump_off_dev = dw.data_ptr(), counts.data_ptr(), off.data_ptr()
+    fin.jump_time_dev, fin.jump_size_dev = None, js.data_ptr()
+    fin.n_marks, fin.n_rows, fin.max_jumps = n_marks, N, 0
+    job = E.make_job(W.model_description('merton'), T, N, L.GRID_UNIFORM, P, 0, 0)
+    E.fill_payoff(job, EuroCall(1.0), math.exp(-0.02 * T))
+    stats = torch.empty(4, dtype=torch.float64, device=dev)
+    E.simulate_fixed(job, fin, dev, stats=stats)
+    torch.cuda.synchronize()
+    ms = _time_events(lambda: E.simulate_fixed(job, fin, dev, stats=stats), reps)
+    s, q, n = E.stats_to_host(stats, 'fixed-increment bench')
+    mean, se = E.mean_and_se(s, q, n)
+    nbytes = 4 * N * P + N * P + 8 * P + 4 * n_marks + 16 * ((P + 63) // 64)
+    return {"workload": "Merton 1-D call, fixed increments from HBM, %d paths x %d steps" % (P, N),
+            "path_steps_s": P * N / (ms * 1e-3), "kernel_ms": ms,
+            "bytes_per_launch": nbytes, "bytes_per_path_step": nbytes / (P * N),
+            "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
+                         "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
+                         "kernel": "k_fixed<1,merton,uniform> + k_finalize_one"},
+            "price": mean, "se": se, "closed_form": MERTON_CLOSED_FORM}
+
+
+def equal_ci_leg(reqs, tune, dev, eps=1e-3, pilot=1 << 13, reps=3):
+    """Every request of the mix priced to a 95% half-width <= eps: a pilot run
+    per request (own Philox stream) sets the path count n = 1.2 (1.96 sd / eps)^2,
+    the run_mc rule with a 20% margin for the pilot's error in sd, capped to
+    [2^10, 2^22]; the same jobs are then priced
+    batched and plain (bit-identical results, so equal confidence intervals)."""
+    import numpy as np
+    import torch
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import workloads as W
+    from sde_mc_amd.batch import BatchRunner, chunks_of
+    pil = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps, pilot,
+                                   r.seed ^ 0x5DEECE66D)) for r in reqs]
+    runner = BatchRunner(max_jobs=len(reqs), partial_chunks=sum(chunks_of(pilot) for _ in reqs),
+                         tune=tune)
+    st = runner.run(pil)
+    sd = np.sqrt(np.maximum((st[:, 1] - st[:, 0] ** 2 / st[:, 2]) / (st[:, 2] - 1), 0.0))
+    n = np.clip(np.ceil(1.2 * (1.96 * sd / eps) ** 2), 1 << 10, 1 << 22).astype(np.int64)
+    sized = [W.Request(r.rid, r.model, r.strike, r.maturity, r.steps, int(k), r.seed)
+             for r, k in zip(reqs, n)]
+    jobs = [W.request_job(r) for r in sized]
+    ps = sum(r.path_steps for r in sized)
+    runner = BatchRunner(max_jobs=len(jobs), partial_chunks=int(sum(chunks_of(k) for k in n)),
+                         tune=tune)
+    ref = runner.run(jobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = runner.run(jobs)
+    tb = (time.perf_counter() - t0) / reps
+    assert np.array_equal(st, ref)
+    for j in jobs[:4]:
+        E.stats_to_host(E.simulate(j, dev))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pst = [E.simulate(j, dev).cpu().numpy() for j in jobs]
+    tp = (time.perf_counter() - t0) / reps
+    assert np.array_equal(np.array(pst), st), "plain and batched results differ"
+    hw = []
+    for row in st:
+        m, se = E.mean_and_se(row[0], row[1], int(row[2]))
+        hw.append(1.96 * se)
+    return {"eps": eps, "requests": len(jobs), "paths_min": int(n.min()), "paths_max": int(n.max()),
+            "path_steps": ps, "batched_path_steps_s": ps / tb, "plain_path_steps_s": ps / tp,
+            "speedup_vs_plain": tp / tb, "max_half_width": float(max(hw)),
+            "requests_over_eps": int(sum(h > eps for h in hw)),
+            "note": "requests capped at 2^22 paths may stay above eps"}
+
+
+def mlmc_leg(tune, dev, reps=5):
+    """MLMC on the 1-D Merton call: levels l=0..6, refinement 2, base 4 steps,
+    P_l = 2^(22 - 10 l / 6) paths, coupled fine/coarse pairs for l >= 1, all
+    levels in one batched launch vs one plain launch per level.  Path-steps
+    count fine steps only."""
+    import numpy as np
+    import torch
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import workloads as W
+    from sde_mc_amd.batch import BatchRunner, chunks_of
+    from sde_mc_amd.options import EuroCall
+    model = W.model_description('merton')
+    T = 1.0
+    jobs, ps = [], 0
+    for l in range(7):
+        P = int(round(2 ** (22 - 10 * l / 6)))
+        N = 4 * 2 ** l
+        job = E.make_job(model, T, N, L.GRID_UNIFORM, P, 7000 + l, 0, coarse_factor=2 if l else 0)
+        E.fill_payoff(job, EuroCall(1.0), math.exp(-0.02 * T))
+        jobs.append(job)
+        ps += P * N
+    runner = BatchRunner(max_jobs=8, partial_chunks=sum(chunks_of(j.n_paths) for j in jobs),
+                         tune=tune)
+    ref = runner.run(jobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = runner.run(jobs)
+    tb = (time.perf_counter() - t0) / reps
+    assert np.array_equal(st, ref)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pst = [E.simulate(j, dev).cpu().numpy() for j in jobs]
+    tp = (time.perf_counter() - t0) / reps
+    assert np.array_equal(np.array(pst), st), "plain and batched MLMC levels differ"
+    est, var = 0.0, 0.0
+    for row in st:
+        m, se = E.mean_and_se(row[0], row[1], int(row[2]))
+        est += m
+        var += se * se
+    return {"levels": 7, "paths": [int(j.n_paths) for j in jobs], "path_steps": ps,
+            "batched_path_steps_s": ps / tb, "plain_path_steps_s": ps / tp,
+            "speedup_vs_plain": tp / tb, "batched_ms": tb * 1e3, "plain_ms": tp * 1e3,
+            "estimate": est, "rmse_se": math.sqrt(var), "closed_form": MERTON_CLOSED_FORM,
+            "error": est - MERTON_CLOSED_FORM}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +230,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the fixed-increment, equal-CI and MLMC legs")
+    ap.add_argument("--tune", default=None,
+                    help="W,B,T launch shape (waves/block, blocks/CU, chunks/tile) instead of "
+                         "autotuning")
+    ap.add_argument("--fixed-only", action="store_true",
+                    help="run only the fixed-increment leg and print its record")
+    ap.add_argument("--probe-copy-mib", type=int, default=0,
+                    help="first run a device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X per process and "
@@ -85,6 +252,16 @@ def main():
     from sde_mc_amd.batch import BatchRunner, chunks_of
 
     dev = E.require_device()
+    if args.probe_copy_mib:
+        src = torch.ones(args.probe_copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.fixed_only:
+        print(json.dumps({"fixed_increment": fixed_increment_leg(dev, args.steps)}))
+        return
     reqs = W.request_mix(args.requests, seed=args.seed)
     jobs = [W.request_job(r) for r in reqs]
     total_ps = sum(r.path_steps for r in reqs)
@@ -93,11 +270,16 @@ def main():
     # autotune the persistent launch shape on a calibration copy of the mix
     # (same requests, path counts capped at 2^16)
     from sde_mc_amd.autotune import autotune
-    calib = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps,
-                                     min(r.paths, 1 << 16), r.seed)) for r in reqs]
-    tune, _ = autotune(calib)
+    from sde_mc_amd.batch import BatchTune
+    if args.tune:
+        tune = BatchTune(*[int(v) for v in args.tune.split(',')])
+    else:
+        calib = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps,
+                                         min(r.paths, 1 << 16), r.seed)) for r in reqs]
+        tune, _ = autotune(calib)
     runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks, tune=tune)
 
+    ref = None
     for _ in range(args.warmup):
         ref = runner.run(jobs)
     torch.cuda.synchronize()
@@ -110,7 +292,7 @@ def main():
     dt = time.perf_counter() - t0
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     batched = total_ps * args.steps / dt
-    assert np.array_equal(st, ref)
+    assert ref is None or np.array_equal(st, ref)
 
     plain = None
     if not args.no_plain:
@@ -144,10 +326,15 @@ def main():
             latency[str(rate)] = {"served": served, "plain": latency_summary(plain_res),
                                   "results_identical": bool(same)}
 
+    extra = {}
+    if not args.no_extra:
+        extra["fixed_increment"] = fixed_increment_leg(dev, args.steps)
+        extra["equal_ci"] = equal_ci_leg(reqs, tune, dev)
+        extra["mlmc"] = mlmc_leg(tune, dev)
+
     cpu = None if args.no_cpu else cpu_baseline(reqs)
 
     achieved = ops / (kern_ms * 1e-3)
-    price_err = []
     out = {
         "metric": METRIC,
         "value": batched,
@@ -177,6 +364,7 @@ def main():
                      "kernel": "k_batch (persistent tile kernel)", "kernel_ms": kern_ms},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

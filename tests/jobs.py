@@ -147,3 +147,47 @@ def fixed_adapted(job, seed=0, extra_rows=0):
         jt = np.full((J, P), np.inf, dtype=np.float32)
     js = mark_sampler(job, rng)(J * P).reshape(J, P) if lam > 0 else np.zeros((J, P), np.float32)
     return FixedInputs(dict(dw=z, jt=jt, js=js), S, J)
+
+
+def reference_kats():
+    """The reference's own known-answer vectors (tests/golden/reference_kats.json)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), 'golden', 'reference_kats.json')) as f:
+        return json.load(f)
+
+
+def payoff_kat_cases():
+    """(kind, strike, terminal state, expected payoff) for every row of the
+    reference's option known-answer tests."""
+    k = reference_kats()
+    out = []
+    for p in k['payoffs']:
+        for x, want in zip(k[p['terminals']], p['expected']):
+            out.append((p['kind'], p['strike'], x, want))
+    return out
+
+
+def frozen_state_job(kind, strike, x):
+    """One path, one uniform step, every coefficient zero and no jumps: the
+    terminal state is x exactly, so the job's payoff is the payoff of x."""
+    job = L.Job()
+    m = job.model
+    m.dim = len(x)
+    m.noise = L.NOISE_DIAG
+    m.jump_law = L.JUMP_LOGNORMAL
+    m.rate = 0.0
+    for i, v in enumerate(x):
+        m.x0[i] = float(v)
+    for i in range(L.MAX_DIM):
+        m.chol[i * L.MAX_DIM + i] = 1.0
+    job.payoff.kind = getattr(L, 'PAYOFF_' + kind)
+    job.payoff.strike = float(strike)
+    job.payoff.scale = 1.0
+    job.payoff.disc = 1.0
+    job.T = 1.0
+    job.steps = 1
+    job.grid = L.GRID_UNIFORM
+    job.n_paths = 1
+    job.seed = 5
+    return job

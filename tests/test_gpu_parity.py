@@ -189,3 +189,56 @@ def test_batch_size_does_not_change_results():
     b1, t1, _, _ = _run_rng(j1)
     b2, t2, _, _ = _run_rng(j2)
     assert np.array_equal(a, np.concatenate([b1, b2]))
+
+
+def test_kernel_payoffs_match_reference_option_kats():
+    # the reference's option known answers (tests/test_options.py:23-50)
+    # through the device path, state frozen at the terminal
+    from tests.jobs import frozen_state_job, payoff_kat_cases
+    dev = E.require_device()
+    for kind, K, x, want in payoff_kat_cases():
+        s, q, n = E.stats_to_host(E.simulate(frozen_state_job(kind, K, x), dev))
+        assert n == 1 and abs(s - want) < 1e-6, (kind, x, s, want)
+
+
+@pytest.mark.parametrize("grid", ['uniform', 'adapted'])
+def test_exact_jumps_match_oracle(grid):
+    # post-step jump coefficient (solvers.py:214-215), fixed inputs and RNG mode
+    P, N = 2048 + 3, 50
+    job = J.job_for(J.merton_sde(rate=5.0), 1.0, N, P, grid=grid, option=EuroCall(1.0),
+                    exact_jumps=True, seed=41)
+    fx = J.fixed_adapted(job, seed=13) if grid == 'adapted' else J.fixed_uniform(job, seed=13)
+    pay, term, ns, s = _run_fixed(job, fx)
+    opay, oterm, ons = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term, oterm) < TOL
+    pay, term, ns, s = _run_rng(job)
+    opay, oterm, ons = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+
+
+def test_double_noise_matches_oracle():
+    # two independent correlated drivers ('indep' diffusion, DoubleGbm)
+    import sde_mc_amd as sm
+    corr = torch.tensor([[1.0, 0.4], [0.4, 1.0]])
+    sde = sm.DoubleGbm(0.03, 0.2, 0.3, torch.tensor([1.0, 1.1]), 2, corr_matrix=corr)
+    P, N = 1500, 40
+    job = J.job_for(sde, 1.0, N, P, option=sm.Basket(1.0), seed=43)
+    fx = J.fixed_uniform(job, seed=14)
+    pay, term, _, s = _run_fixed(job, fx)
+    opay, oterm, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term, oterm) < TOL
+    pay, term, _, s = _run_rng(job)
+    opay, oterm, _ = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+
+
+def test_put_and_geometric_basket_payoffs_match_oracle():
+    from tests.jobs import frozen_state_job
+    dev = E.require_device()
+    for kind, K, x in (('EURO_PUT', 1.0, [0.4]), ('EURO_PUT', 1.0, [1.3]),
+                       ('BASKET_GEO', 0.5, [2.0, 0.7]), ('BASKET_GEO', 0.5, [0.9, 1.4, 1.1])):
+        job = frozen_state_job(kind, K, x)
+        s, q, n = E.stats_to_host(E.simulate(job, dev))
+        want = oracle.simulate(job, 0, 1)[0][0]
+        assert n == 1 and abs(s - want) < 1e-6 * (1 + abs(want)), (kind, x, s, want)

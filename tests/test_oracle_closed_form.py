@@ -60,3 +60,22 @@ def test_stulz_limits():
     # rho -> 1 with equal vols: max of two identical assets is one asset
     v = cf.call_max_quadrature(1, 1, 1, 1, 0.02, 0.2, 0.2, 0.999999)
     assert abs(v - cf.bs_call(1, 1, 1, 0.02, 0.2)) < 2e-4
+
+
+def test_golden_fixture_closed_forms():
+    from tests.jobs import reference_kats
+    k = reference_kats()['closed_forms']
+    for name, rel in (('bs_call', 1e-5), ('bs_binary_aon', 1e-9), ('merton_call', 1e-13)):
+        assert math.isclose(getattr(cf, name)(*k[name]['args']), k[name]['expected'], rel_tol=rel)
+
+
+def test_oracle_payoffs_match_reference_option_kats():
+    # the reference's option known answers (tests/test_options.py:23-50),
+    # through the oracle's whole path with the state frozen at the terminal
+    import numpy as np
+    import oracle
+    from tests.jobs import frozen_state_job, payoff_kat_cases
+    for kind, K, x, want in payoff_kat_cases():
+        pay, term, _ = oracle.simulate(frozen_state_job(kind, K, x), 0, 1)
+        assert np.allclose(term[0, :, 0], np.float32(x)), (kind, x)
+        assert abs(pay[0] - want) < 1e-6, (kind, x, pay[0], want)

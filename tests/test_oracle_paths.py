@@ -82,6 +82,55 @@ def test_single_forced_jump_uniform():
     assert np.allclose(term[0, 0], np.prod(f, axis=0), rtol=1e-13)
 
 
+def test_exact_jump_rule_uses_post_step_state():
+    # exact_jumps: the jump coefficient is taken after the diffusion step
+    # (solvers.py:214-215): x_4 = x_3 (1 + mu h + sigma dW_3)(1 + J)
+    P, N = 4, 10
+    job = J.job_for(J.merton_sde(), 1.0, N, P, exact_jumps=True)
+    fx = J.fixed_uniform(job, seed=3)
+    fx.a['counts'][:] = 0
+    fx.a['counts'][3, :] = 1
+    fx.a['off'] = np.arange(P + 1, dtype=np.int64)
+    fx.a['js'] = np.full(P, 0.5, dtype=np.float32)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    a1 = job.model.a1[0]
+    dw = fx.a['dw'][:, 0, :].astype(np.float64)
+    f = 1 + a1 / N + 0.2 * dw
+    f[3] *= 1.5
+    assert np.allclose(term[0, 0], np.prod(f, axis=0), rtol=1e-13)
+
+
+def test_double_noise_product_formula():
+    # 'indep' diffusion with two independent drivers (DoubleGbm, sde.py:222-255):
+    # x_N = x_0 prod(1 + mu h + s1 dW_n + s2 dB_n), dW and dB the two dw rows
+    import sde_mc_amd as sm
+    P, N = 12, 30
+    job = J.job_for(sm.DoubleGbm(0.03, 0.2, 0.4, torch.tensor([1.0]), 1), 1.0, N, P)
+    fx = J.fixed_uniform(job, seed=8)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    dw = fx.a['dw'].astype(np.float64)
+    want = np.prod(1 + 0.03 / N + 0.2 * dw[:, 0, :] + 0.4 * dw[:, 1, :], axis=0)
+    assert np.allclose(term[0, 0], want, rtol=1e-13)
+
+
+def test_put_call_parity_and_basket_invariants():
+    # call(x) - put(x) = x - K with the call pinned by the reference's known
+    # answers; geometric basket <= arithmetic basket, equal on equal components
+    from tests.jobs import frozen_state_job
+    for x in (0.3, 0.9, 1.0, 1.7):
+        c = oracle.simulate(frozen_state_job('EURO_CALL', 1.0, [x]), 0, 1)[0][0]
+        p = oracle.simulate(frozen_state_job('EURO_PUT', 1.0, [x]), 0, 1)[0][0]
+        assert abs((c - p) - (x - 1.0)) < 1e-12
+    for x in ([1.5, 1.5], [2.0, 0.7], [0.9, 1.4, 1.1]):
+        g = oracle.simulate(frozen_state_job('BASKET_GEO', 0.5, x), 0, 1)[0][0]
+        a = oracle.simulate(frozen_state_job('BASKET_ARITH', 0.5, x), 0, 1)[0][0]
+        assert g <= a + 1e-12
+        if len(set(x)) == 1:
+            assert abs(g - a) < 1e-12
+        else:
+            assert abs(g - (np.exp(np.mean(np.log(x))) - 0.5)) < 1e-12
+
+
 def test_adapted_grid_restarts_at_jump():
     # h = 0.25, one jump at t = 0.3: steps end at .25, .3 (jump), .55, .8, 1.0
     P = 3
