@@ -268,14 +268,14 @@ def main():
     ops = sum(r.path_steps * W.valu_ops_per_path_step(r.model, r.steps, r.maturity) for r in reqs)
     chunks = sum(chunks_of(r.paths) for r in reqs)
     # autotune the persistent launch shape on a calibration copy of the mix
-    # (same requests, path counts capped at 2^16)
+    # (same requests, path counts capped at 2^18)
     from sde_mc_amd.autotune import autotune
     from sde_mc_amd.batch import BatchTune
     if args.tune:
         tune = BatchTune(*[int(v) for v in args.tune.split(',')])
     else:
         calib = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps,
-                                         min(r.paths, 1 << 16), r.seed)) for r in reqs]
+                                         min(r.paths, 1 << 18), r.seed)) for r in reqs]
         tune, _ = autotune(calib)
     runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks, tune=tune)
 
@@ -354,7 +354,8 @@ def main():
                    "launch": "batched persistent kernel, graph-replayed",
                    "tune": {"waves_per_block": tune.waves_per_block,
                             "blocks_per_cu": tune.blocks_per_cu,
-                            "tile_chunks": tune.tile_chunks}},
+                            "tile_chunks": tune.tile_chunks,
+                            "concurrent_groups": tune.concurrent_groups}},
         "plain_value": plain,
         "speedup_vs_plain": (batched / plain) if plain else None,
         "latency": latency,

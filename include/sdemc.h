@@ -292,14 +292,18 @@ typedef struct sdemc_batch_dev {
 } sdemc_batch_dev;
 
 typedef struct sdemc_batch_tune {
-  int32_t waves_per_block; /* 1, 2, 4 or 8                   */
-  int32_t blocks_per_cu;   /* persistent workgroups per CU   */
-  int32_t reserved0;
-  int32_t reserved1;
+  int32_t waves_per_block;   /* 1, 2, 4 or 8                                  */
+  int32_t blocks_per_cu;     /* persistent workgroups per CU                  */
+  int32_t concurrent_groups; /* 0: group kernels run one after another on the
+                                stream; 1: forked onto side streams (default
+                                priority) and joined back, so a group's tail
+                                overlaps the next group                        */
+  int32_t reserved;
 } sdemc_batch_tune;
 
 /* One batched launch: zero the tile counters, run the group kernels,
- * finalise the flagged slots.  Everything is stream-ordered. */
+ * finalise the flagged slots.  Everything is ordered after earlier work on
+ * `stream`, and later work on `stream` waits for all of it. */
 int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream);
 
 /* The same launches captured once into a HIP graph and replayed by

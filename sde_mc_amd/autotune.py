@@ -16,8 +16,13 @@ import torch
 from . import _lib as L
 from .batch import BatchRunner, BatchTune, chunks_of, job_class
 
-CANDIDATES = [BatchTune(w, b, t) for w in (2, 4, 8) for b in (2, 4, 8, 16) for t in (1, 2, 4)
-              if 2 <= w * b <= 64]
+# A CU has 4 SIMDs and the tile kernels fit at most 7 wavefronts per SIMD
+# (their scalar-register use), so the shapes swept are workgroups of 2, 4 or
+# 8 wavefronts with 3..7 resident wavefronts per SIMD in all, and 1 or 2
+# chunks per tile.
+WAVES_PER_SIMD = (3, 4, 5, 6, 7)
+CANDIDATES = [BatchTune(w, 4 * s // w, t) for w in (2, 4, 8) for s in WAVES_PER_SIMD
+              for t in (1, 2) if (4 * s) % w == 0]
 
 
 def cache_path():

@@ -71,25 +71,28 @@ def job_class(job):
 
 
 class BatchTune:
-    """Launch shape of the persistent kernel: wavefronts per workgroup,
-    resident workgroups per CU, chunks (64 paths each) per tile."""
+    """Launch shape of the persistent kernels: wavefronts per workgroup,
+    resident workgroups per CU, least chunks (64 paths each) per tile, and
+    whether a round's class groups run concurrently."""
 
-    def __init__(self, waves_per_block=4, blocks_per_cu=2, tile_chunks=1):
+    def __init__(self, waves_per_block=4, blocks_per_cu=7, tile_chunks=1, concurrent_groups=1):
         self.waves_per_block = int(waves_per_block)
         self.blocks_per_cu = int(blocks_per_cu)
         self.tile_chunks = int(tile_chunks)
+        self.concurrent_groups = int(concurrent_groups)
 
     def c(self):
         t = L.BatchTune()
         t.waves_per_block = self.waves_per_block
         t.blocks_per_cu = self.blocks_per_cu
+        t.concurrent_groups = self.concurrent_groups
         return t
 
     def key(self):
-        return (self.waves_per_block, self.blocks_per_cu, self.tile_chunks)
+        return (self.waves_per_block, self.blocks_per_cu, self.tile_chunks, self.concurrent_groups)
 
     def __repr__(self):
-        return 'BatchTune(wpb=%d, bpc=%d, tile_chunks=%d)' % self.key()
+        return 'BatchTune(wpb=%d, bpc=%d, tile_chunks=%d, concurrent_groups=%d)' % self.key()
 
 
 class RoundPlan:
@@ -108,11 +111,19 @@ class RoundPlan:
         self.segs, self.hdr, self.groups, self.tiles = segs, hdr, tuple(groups), int(tiles)
 
 
-def plan_segments(slots, firsts, counts, costs, tile_chunks, classes=None):
+# Least work of one tile, in job_cost units (about 300 steps of a 1-D chunk):
+# a wavefront takes a tile index from its group's counter with one atomic, so
+# short jobs (e.g. MLMC level 0, 4 steps) get several chunks per tile.
+MIN_TILE_COST = 2.0e4
+
+
+def plan_segments(slots, firsts, counts, costs, tile_chunks, classes=None,
+                  min_tile_cost=MIN_TILE_COST):
     """Group the segments by kernel class (generic when `classes` is None) and
     lay out consecutive tile ranges within each group; returns a RoundPlan.
-    Groups launch one after another, the costliest group first; when more
-    classes are present than a launch has groups, the cheapest dedicated
+    A segment's tiles hold max(tile_chunks, ceil(min_tile_cost / cost))
+    chunks.  Groups launch one after another, the costliest group first; when
+    more classes are present than a launch has groups, the cheapest dedicated
     groups fold into the generic one."""
     slots = np.asarray(slots, dtype=np.int64)
     firsts = np.asarray(firsts, dtype=np.int64)
@@ -136,7 +147,9 @@ def plan_segments(slots, firsts, counts, costs, tile_chunks, classes=None):
     for g, c in enumerate(groups):
         rank[c] = g
     order = np.lexsort((-costs, rank[cls])) if slots.size else np.zeros(0, np.int64)
-    per = np.full(order.size, max(1, int(tile_chunks)), dtype=np.int64)
+    per = np.maximum(max(1, int(tile_chunks)),
+                     np.ceil(float(min_tile_cost) / np.maximum(costs[order], 1e-30))).astype(np.int64)
+    per = np.minimum(per, 1 << 20)
     nt = (counts[order] + per - 1) // per
     grp = rank[cls[order]]
     hdr = np.zeros((L.MAX_GROUPS, 4), dtype=np.int32)
@@ -294,7 +307,7 @@ class BatchRunner:
         stream = E.stream_ptr()
         if self.use_graph:
             key = (id(buf), groups, tune.waves_per_block, tune.blocks_per_cu,
-                   self.partials.data_ptr())
+                   tune.concurrent_groups, self.partials.data_ptr())
             plan = self._plans.get(key)
             if plan is None:
                 h = ctypes.c_void_p()

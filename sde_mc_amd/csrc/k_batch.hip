@@ -6,6 +6,9 @@
 // involvement; the host orders segments costliest-first.
 #include <string.h>
 
+#include <map>
+#include <mutex>
+
 #include "sdemc_kernels.h"
 
 namespace sdemc {
@@ -233,23 +236,89 @@ static void launch_group(const sdemc_batch_dev& b, int g, unsigned grid, int wpb
                      b.counter_dev + g, b.partial_chunks, b.max_jobs, b.max_segments);
 }
 
-static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
+// Streams and events that fork a round's group kernels off the caller's
+// stream and join them back (default priority, non-blocking).
+struct Fork {
+  hipStream_t side[SDEMC_MAX_GROUPS - 1];
+  hipEvent_t fork;
+  hipEvent_t join[SDEMC_MAX_GROUPS - 1];
+  bool ok;
+};
+
+static bool fork_create(Fork& f) {
+  memset(&f, 0, sizeof(f));
+  f.ok = hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; f.ok && i < SDEMC_MAX_GROUPS - 1; ++i) {
+    f.ok = hipStreamCreateWithFlags(&f.side[i], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming) == hipSuccess;
+  }
+  return f.ok;
+}
+
+static void fork_destroy(Fork& f) {
+  for (int i = 0; i < SDEMC_MAX_GROUPS - 1; ++i) {
+    if (f.join[i]) hipEventDestroy(f.join[i]);
+    if (f.side[i]) hipStreamDestroy(f.side[i]);
+  }
+  if (f.fork) hipEventDestroy(f.fork);
+  memset(&f, 0, sizeof(f));
+}
+
+// The process-wide fork of each device for launches outside a graph.
+static Fork* device_fork() {
+  static std::mutex mu;
+  static std::map<int, Fork> forks;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = forks.find(dev);
+  if (it == forks.end()) {
+    Fork f;
+    if (!fork_create(f)) {
+      fork_destroy(f);
+      return nullptr;
+    }
+    it = forks.emplace(dev, f).first;
+  }
+  return &it->second;
+}
+
+static void launch_class(const sdemc_batch_dev& b, int g, unsigned grid, int wpb,
                          hipStream_t s) {
+  switch (b.group_class[g]) {
+    case 0: launch_group<0>(b, g, grid, wpb, s); break;
+    case 1: launch_group<1>(b, g, grid, wpb, s); break;
+    case 2: launch_group<2>(b, g, grid, wpb, s); break;
+    case 3: launch_group<3>(b, g, grid, wpb, s); break;
+    case 4: launch_group<4>(b, g, grid, wpb, s); break;
+    case 5: launch_group<5>(b, g, grid, wpb, s); break;
+    case 6: launch_group<6>(b, g, grid, wpb, s); break;
+    case 7: launch_group<7>(b, g, grid, wpb, s); break;
+    default: launch_group<SDEMC_CLASS_GENERIC>(b, g, grid, wpb, s); break;
+  }
+}
+
+// Zero the tile counters, run the group kernels -- one after another on s,
+// or (fork != null) group 0 on s and group g > 0 on fork->side[g - 1], all
+// after the counter reset and joined back into s -- then finalise.
+static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune, hipStream_t s,
+                         Fork* fork) {
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, b.counter_dev, b.n_groups);
   const int wpb = tune.waves_per_block;
   const unsigned grid = (unsigned)(cu_count_cached() * tune.blocks_per_cu);
-  for (int g = 0; g < b.n_groups; ++g) {
-    switch (b.group_class[g]) {
-      case 0: launch_group<0>(b, g, grid, wpb, s); break;
-      case 1: launch_group<1>(b, g, grid, wpb, s); break;
-      case 2: launch_group<2>(b, g, grid, wpb, s); break;
-      case 3: launch_group<3>(b, g, grid, wpb, s); break;
-      case 4: launch_group<4>(b, g, grid, wpb, s); break;
-      case 5: launch_group<5>(b, g, grid, wpb, s); break;
-      case 6: launch_group<6>(b, g, grid, wpb, s); break;
-      case 7: launch_group<7>(b, g, grid, wpb, s); break;
-      default: launch_group<SDEMC_CLASS_GENERIC>(b, g, grid, wpb, s); break;
+  if (fork && b.n_groups > 1) {
+    if (hipEventRecord(fork->fork, s) != hipSuccess) return SDEMC_ERR_LAUNCH;
+    for (int g = 1; g < b.n_groups; ++g)
+      if (hipStreamWaitEvent(fork->side[g - 1], fork->fork, 0) != hipSuccess) return SDEMC_ERR_LAUNCH;
+    launch_class(b, 0, grid, wpb, s);
+    for (int g = 1; g < b.n_groups; ++g) {
+      launch_class(b, g, grid, wpb, fork->side[g - 1]);
+      if (hipEventRecord(fork->join[g - 1], fork->side[g - 1]) != hipSuccess) return SDEMC_ERR_LAUNCH;
     }
+    for (int g = 1; g < b.n_groups; ++g)
+      if (hipStreamWaitEvent(s, fork->join[g - 1], 0) != hipSuccess) return SDEMC_ERR_LAUNCH;
+  } else {
+    for (int g = 0; g < b.n_groups; ++g) launch_class(b, g, grid, wpb, s);
   }
   hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s,
                      (const KJob*)b.jobs_dev, b.finalize_dev, b.partials_dev, b.chunk_flags_dev,
@@ -297,8 +366,14 @@ int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_ba
 int sdemc_batch_launch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune, void* stream) {
   int st = check_batch(b, tune);
   if (st != SDEMC_OK) return st;
-  enqueue_batch(*b, *tune, (hipStream_t)stream);
+  Fork* fork = nullptr;
+  if (tune->concurrent_groups && b->n_groups > 1) {
+    fork = device_fork();
+    if (!fork) return SDEMC_ERR_LAUNCH;
+  }
+  st = enqueue_batch(*b, *tune, (hipStream_t)stream, fork);
   const hipError_t e = hipGetLastError();
+  if (st != SDEMC_OK) return st;
   return e == hipSuccess ? SDEMC_OK : SDEMC_ERR_LAUNCH;
 }
 
@@ -311,15 +386,24 @@ int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tu
   (void)cu_count_cached();
   hipStream_t cs;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return SDEMC_ERR_LAUNCH;
+  Fork fork;
+  const bool forked = tune->concurrent_groups && b->n_groups > 1;
+  if (forked && !fork_create(fork)) {
+    fork_destroy(fork);
+    hipStreamDestroy(cs);
+    return SDEMC_ERR_LAUNCH;
+  }
   sdemc_batch_plan* p = new sdemc_batch_plan();
   p->graph = nullptr;
   p->exec = nullptr;
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    enqueue_batch(*b, *tune, cs);
+    const int qs = enqueue_batch(*b, *tune, cs, forked ? &fork : nullptr);
     e = hipStreamEndCapture(cs, &p->graph);
+    if (e == hipSuccess && qs != SDEMC_OK) e = hipErrorUnknown;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (forked) fork_destroy(fork);
   hipStreamDestroy(cs);
   if (e != hipSuccess) {
     if (p->exec) hipGraphExecDestroy(p->exec);

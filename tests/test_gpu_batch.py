@@ -40,7 +40,8 @@ def test_batch_equals_plain_bitwise():
     jobs = _mix()
     plain = np.array([_plain(j) for j in jobs])
     r = BatchRunner(max_jobs=64)
-    for tune in (BatchTune(4, 2, 1), BatchTune(1, 4, 4), BatchTune(8, 1, 2)):
+    for tune in (BatchTune(4, 2, 1), BatchTune(1, 4, 4), BatchTune(8, 1, 2),
+                 BatchTune(4, 7, 1, concurrent_groups=0), BatchTune(2, 14, 2)):
         got = r.run(jobs, tune=tune)
         assert np.array_equal(got, plain), tune
 
@@ -64,6 +65,8 @@ def test_graph_replays_are_consistent():
         assert np.array_equal(again, first)
     r2 = BatchRunner(max_jobs=64, use_graph=False)
     assert np.array_equal(r2.run(jobs), first)
+    r3 = BatchRunner(max_jobs=64, use_graph=False, tune=BatchTune(concurrent_groups=0))
+    assert np.array_equal(r3.run(jobs), first)
 
 
 def test_generic_class_with_dedicated_classes():

@@ -36,7 +36,8 @@ def test_segments_cover_every_chunk_once_costliest_first():
     costs = rng.uniform(1, 100, n)
     for classes in (None, rng.integers(0, 9, n), np.full(n, 3)):
         for per in (1, 2, 4, 7):
-            plan = plan_segments(np.arange(n), firsts, counts, costs, per, classes)
+            plan = plan_segments(np.arange(n), firsts, counts, costs, per, classes,
+                                 min_tile_cost=50.0 * per)
             segs = plan.segs
             assert len(segs) == int((counts > 0).sum())
             assert len(plan.groups) <= L.MAX_GROUPS
@@ -57,6 +58,16 @@ def test_segments_cover_every_chunk_once_costliest_first():
             # expand tiles -> chunks: exactly the requested ranges
             want = {(i, firsts[i] + c): 1 for i in range(n) for c in range(counts[i])}
             assert _expand(plan) == want
+
+
+def test_short_jobs_get_several_chunks_per_tile():
+    from sde_mc_amd.batch import MIN_TILE_COST
+    costs = np.array([MIN_TILE_COST * 2, MIN_TILE_COST / 8, MIN_TILE_COST / 7.5])
+    plan = plan_segments([0, 1, 2], [0, 0, 0], [100, 100, 100], costs, 1)
+    per = dict(zip(plan.segs[:, 0], plan.segs[:, 3]))
+    assert per == {0: 1, 1: 8, 2: 8}
+    plan = plan_segments([0, 1, 2], [0, 0, 0], [100, 100, 100], costs, 4)
+    assert dict(zip(plan.segs[:, 0], plan.segs[:, 3])) == {0: 4, 1: 8, 2: 8}
 
 
 def test_nine_classes_fold_into_eight_groups():
