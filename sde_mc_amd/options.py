@@ -1,72 +1,90 @@
-"""Payoffs, discounting and closed-form prices (sde_mc/options.py).
+"""Payoffs, discounting and closed-form prices (API of sde_mc/options.py).
 
-Each Option is a torch functor on terminal states, exactly as in the
-reference; the HIP kernels evaluate the recognised ones in-kernel from
-:meth:`Option.kernel_payoff`.  Other Option subclasses are applied in torch to
-the terminal states the kernel returns.
+An Option maps terminal states x (P, d) to payoffs (P,): the state is first
+exponentiated when ``log`` is set and multiplied by ``discount``, then the
+payoff is applied.  The HIP kernels evaluate the payoffs listed in
+``_IN_KERNEL`` themselves (``Option.kernel_payoff`` fills the C descriptor);
+any other Option, including a subclass that overrides ``payoff``, is applied
+in torch to the terminal states the kernel returns.
 """
+import math
 from abc import ABC, abstractmethod
-from math import factorial
 
 import numpy as np
 import torch
-from scipy.integrate import quad
-from scipy.stats import lognorm, norm
+from scipy.special import ndtr
 
 from . import _lib as L
 
 
+# ---- closed forms --------------------------------------------------------------
+def _d1_d2(spot, strike, expiry, r, sigma):
+    vol = sigma * math.sqrt(expiry)
+    d1 = (math.log(spot / strike) + (r + 0.5 * sigma * sigma) * expiry) / vol
+    return d1, d1 - vol
+
+
 def bs_binary_aon(spot, strike, expiry, r, sigma):
-    """Black-Scholes asset-or-nothing price, as normalised in options.py:8-32."""
-    upper = (np.log(spot / strike) + (r + 0.5 * sigma * sigma) * expiry) / (np.sqrt(expiry) * sigma)
-    return quad(lambda z: np.exp(-0.5 * z * z), -np.inf, upper)[0] / np.sqrt(2 * np.pi)
+    """N(d1) of Black-Scholes: the asset-or-nothing price per unit of spot, the
+    normalisation of options.py:8-32."""
+    return float(ndtr(_d1_d2(spot, strike, expiry, r, sigma)[0]))
 
 
 def bs_call(spot, strike, expiry, r, sigma):
     """Black-Scholes European call (options.py:35-58)."""
-    d1 = (np.log(spot / strike) + (r + sigma ** 2 / 2) * expiry) / (sigma * np.sqrt(expiry))
-    d2 = d1 - sigma * np.sqrt(expiry)
-    return spot * norm.cdf(d1) - strike * np.exp(-r * expiry) * norm.cdf(d2)
+    d1, d2 = _d1_d2(spot, strike, expiry, r, sigma)
+    return float(spot * ndtr(d1) - strike * math.exp(-r * expiry) * ndtr(d2))
 
 
 def merton_call(spot, strike, expiry, r, sigma, alpha, gamma, rate, terms=40):
-    """Merton jump-diffusion call as a Poisson mixture of Black-Scholes prices
-    (options.py:61-99): sum_k e^{-lambda' T} (lambda' T)^k / k! BS(r_k, sigma_k)
-    with lambda' = lambda (1 + kbar), r_k = r - lambda kbar + k ln(1 + kbar) / T and
-    sigma_k^2 = sigma^2 + k gamma^2 / T.  Uses math.factorial (numpy >= 2 has
-    no np.math)."""
-    beta = np.exp(alpha + 0.5 * gamma * gamma) - 1
-    total = 0.0
-    for k in range(terms):
-        r_k = r - rate * beta + (k * np.log(beta + 1)) / expiry
-        sigma_k = np.sqrt(sigma ** 2 + (k * gamma ** 2) / expiry)
-        lt = (beta + 1) * rate * expiry
-        total += np.exp(-lt) * lt ** k / factorial(k) * bs_call(spot, strike, expiry, r_k, sigma_k)
-    return total
+    """Merton (1976) call with log-normal jumps log(1 + J) ~ N(alpha, gamma^2):
+    conditional on n jumps the price is Black-Scholes with variance
+    sigma^2 + n gamma^2 / T and rate r - lambda kbar + n log(1 + kbar) / T,
+    kbar = E[J]; the n-th term is weighted by the Poisson(lambda (1 + kbar) T)
+    probability of n (options.py:61-99, first `terms` terms)."""
+    kbar = math.expm1(alpha + 0.5 * gamma * gamma)
+    mass = rate * (1.0 + kbar) * expiry
+    price = 0.0
+    for n in range(terms):
+        if mass > 0:
+            weight = math.exp(n * math.log(mass) - mass - math.lgamma(n + 1))
+        else:
+            weight = 1.0 if n == 0 else 0.0
+        vol_n = math.sqrt(sigma * sigma + n * gamma * gamma / expiry)
+        r_n = r - rate * kbar + n * math.log1p(kbar) / expiry
+        price += weight * bs_call(spot, strike, expiry, r_n, vol_n)
+    return price
 
 
 def bs_digital_call(spot, strike, expiry, r, sigma):
-    """Black-Scholes cash-or-nothing digital (options.py:102-125)."""
-    mn = np.log(spot) + (r - 0.5 * sigma * sigma) * expiry
-    sd = sigma * np.sqrt(expiry)
-    return (1 - lognorm.cdf(strike, s=sd, scale=np.exp(mn))) * np.exp(-r * expiry)
+    """Cash-or-nothing call e^{-rT} P(S_T > K) under Black-Scholes
+    (options.py:102-125)."""
+    return float(math.exp(-r * expiry) * ndtr(_d1_d2(spot, strike, expiry, r, sigma)[1]))
 
 
 def bs_asian_call(spot, strike, expiry, r, sigma):
-    """Geometric-average Asian call under Black-Scholes (options.py:128-153)."""
-    sig_g = sigma / np.sqrt(3)
-    b = 0.5 * (r - 0.5 * sig_g ** 2)
-    d1 = (np.log(spot / strike) + (b + sig_g ** 2 / 2) * expiry) / (sig_g * np.sqrt(expiry))
-    d2 = d1 - sig_g * np.sqrt(expiry)
-    return spot * np.exp((b - r) * expiry) * norm.cdf(d1) - strike * np.exp(-r * expiry) * norm.cdf(d2)
+    """Continuously monitored geometric-average Asian call under Black-Scholes
+    (Kemna-Vorst): the average is log-normal with volatility sigma / sqrt(3)
+    and cost of carry (r - sigma^2 / 6) / 2 (options.py:128-153)."""
+    vol_g = sigma / math.sqrt(3.0)
+    carry = 0.5 * (r - 0.5 * vol_g * vol_g)
+    sd = vol_g * math.sqrt(expiry)
+    d1 = (math.log(spot / strike) + (carry + 0.5 * vol_g * vol_g) * expiry) / sd
+    fwd = spot * math.exp((carry - r) * expiry)
+    return float(fwd * ndtr(d1) - strike * math.exp(-r * expiry) * ndtr(d1 - sd))
 
 
 def _f(v):
     return float(v.detach().double().cpu()) if torch.is_tensor(v) else float(v)
 
 
+def _floor_at_zero(v):
+    return torch.clamp(v, min=0.0)
+
+
+# ---- options ---------------------------------------------------------------------
 class Option(ABC):
-    """Payoff of a transformed terminal value: payoff(discount * (exp(x) if log else x))
+    """payoff(discount * (exp(x) if log else x)) on terminal states x (P, d)
     (options.py:156-176)."""
 
     kernel_kind = None
@@ -76,74 +94,66 @@ class Option(ABC):
         self.discount = discount
 
     def transform(self, x):
-        spot = torch.exp(x) if self.log else x
-        return self.discount * spot
+        return self.discount * (x.exp() if self.log else x)
 
     @abstractmethod
     def payoff(self, x):
-        pass
+        """Payoff of transformed states (P, d) -> (P,)."""
 
     def __call__(self, x):
         return self.payoff(self.transform(x))
 
     def kernel_payoff(self, p):
-        """Fill a ctypes sdemc_payoff; returns False if there is no in-kernel form."""
-        owner = next(c for c in type(self).__mro__ if 'payoff' in c.__dict__)
-        if self.kernel_kind is None or owner not in _PAYOFF_IMPL:
+        """Fill a ctypes sdemc_payoff; False when the kernels cannot evaluate
+        this payoff (it is then applied in torch)."""
+        impl = next(c for c in type(self).__mro__ if 'payoff' in c.__dict__)
+        if self.kernel_kind is None or impl not in _IN_KERNEL:
             return False
         p.kind = self.kernel_kind
-        p.log = 1 if self.log else 0
+        p.log = int(bool(self.log))
         p.strike = _f(getattr(self, 'strike', 0.0))
         p.scale = _f(self.discount)
         return True
 
 
-class EuroCall(Option):
-    """European call (options.py:179-199)."""
+class _Struck(Option):
+    def __init__(self, strike, log=False, discount=1):
+        super().__init__(log, discount)
+        self.strike = strike
+
+
+class EuroCall(_Struck):
+    """(S_0 - K)^+ (options.py:179-199)."""
     kernel_kind = L.PAYOFF_EURO_CALL
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        x = x[:, 0]
-        return torch.where(x > self.strike, x - self.strike,
-                           torch.tensor(0., dtype=x.dtype, device=x.device))
+        return _floor_at_zero(x[:, 0] - self.strike)
 
 
-class EuroPut(Option):
-    """European put (options.py:202-222)."""
+class EuroPut(_Struck):
+    """(K - S_0)^+ (options.py:202-222)."""
     kernel_kind = L.PAYOFF_EURO_PUT
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        x = x[:, 0]
-        return torch.where(x < self.strike, self.strike - x,
-                           torch.tensor(0., dtype=x.dtype, device=x.device))
+        return _floor_at_zero(self.strike - x[:, 0])
 
 
-class BinaryAoN(Option):
-    """Binary asset-or-nothing (options.py:225-244)."""
+class BinaryAoN(_Struck):
+    """S_0 1{S_0 >= K} (options.py:225-244)."""
     kernel_kind = L.PAYOFF_BINARY_AON
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        x = x[:, 0]
-        return torch.where(x >= self.strike, x, torch.tensor(0, dtype=x.dtype, device=x.device))
+        s = x[:, 0]
+        return s * (s >= self.strike).to(s.dtype)
 
 
 class Basket(Option):
-    """Arithmetic or geometric basket call (options.py:247-259)."""
+    """Call on the arithmetic or geometric mean of the components
+    (options.py:247-259)."""
 
     def __init__(self, strike, average_type='arithmetic', log=False, discount=1):
-        assert average_type in ['arithmetic', 'geometric']
+        if average_type not in ('arithmetic', 'geometric'):
+            raise AssertionError("average_type must be 'arithmetic' or 'geometric'")
         super().__init__(log, discount)
         self.strike = strike
         self.average_type = average_type
@@ -151,40 +161,33 @@ class Basket(Option):
                             else L.PAYOFF_BASKET_GEO)
 
     def payoff(self, x):
-        spot = x.mean(1) if self.average_type == 'arithmetic' else torch.exp(torch.log(x).mean(1))
-        return torch.where(spot > self.strike, spot - self.strike,
-                           torch.tensor(0., dtype=spot.dtype, device=spot.device))
+        if self.average_type == 'arithmetic':
+            avg = x.mean(dim=1)
+        else:
+            avg = x.log().mean(dim=1).exp()
+        return _floor_at_zero(avg - self.strike)
 
 
-class Rainbow(Option):
-    """Call on the maximum (options.py:262-272)."""
+class Rainbow(_Struck):
+    """Call on the maximum, (max_i S_i - K)^+ (options.py:262-272)."""
     kernel_kind = L.PAYOFF_RAINBOW
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        spot = x.max(1).values
-        return torch.where(spot > self.strike, spot - self.strike,
-                           torch.tensor(0., dtype=spot.dtype, device=spot.device))
+        return _floor_at_zero(x.amax(dim=1) - self.strike)
 
 
-class Digital(Option):
-    """Digital (cash-or-nothing) call (options.py:275-284)."""
+class Digital(_Struck):
+    """1{S_0 > K} (options.py:275-284)."""
     kernel_kind = L.PAYOFF_DIGITAL
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        x = x[:, 0]
-        return torch.where(x > self.strike, torch.ones_like(x), torch.zeros_like(x))
+        s = x[:, 0]
+        return (s > self.strike).to(s.dtype)
 
 
 class AsianCall(Option):
-    """Asian call on the running integral in component 1 (options.py:287-298)."""
+    """Call on the time average A_T / T held in component 1, exponentiated
+    when ``log`` (options.py:287-298)."""
 
     def __init__(self, time_interval, strike, log=False, discount=1):
         super().__init__(log, discount)
@@ -192,36 +195,26 @@ class AsianCall(Option):
         self.strike = strike
 
     def payoff(self, x):
-        spot = torch.exp(x[:, 1] / self.time_interval) if self.log else x[:, 1] / self.time_interval
-        return torch.where(spot > self.strike, spot - self.strike,
-                           torch.tensor(0., dtype=spot.dtype, device=spot.device))
+        avg = x[:, 1] / self.time_interval
+        if self.log:
+            avg = avg.exp()
+        return _floor_at_zero(avg - self.strike)
 
 
-class HestonRainbow(Option):
-    """Call on the max of the even components (options.py:301-311)."""
-
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
+class HestonRainbow(_Struck):
+    """Call on the maximum of the price components 0, 2, 4, ... of a
+    price/variance state (options.py:301-311)."""
 
     def payoff(self, x):
-        even = torch.tensor([i for i in range(len(x[0])) if not i % 2], device=x.device)
-        spot = torch.index_select(x, 1, even).max(1).values
-        return torch.where(spot > self.strike, spot - self.strike,
-                           torch.tensor(0., dtype=spot.dtype, device=spot.device))
+        return _floor_at_zero(x[:, 0::2].amax(dim=1) - self.strike)
 
 
-class BestOf(Option):
+class BestOf(_Struck):
     """max(max_i S_i, K) (options.py:314-321)."""
     kernel_kind = L.PAYOFF_BEST_OF
 
-    def __init__(self, strike, log=False, discount=1):
-        super().__init__(log, discount)
-        self.strike = strike
-
     def payoff(self, x):
-        m = torch.max(x, dim=1).values
-        return torch.maximum(m, self.strike * torch.ones_like(m))
+        return torch.clamp(x.amax(dim=1), min=float(self.strike))
 
 
 class ConstantShortRate:
@@ -231,11 +224,8 @@ class ConstantShortRate:
         self.r = r
 
     def __call__(self, t):
-        if not torch.is_tensor(t):
-            t = torch.tensor(t)
-        return torch.exp(-t * self.r)
+        return torch.exp(-self.r * torch.as_tensor(t))
 
 
-# payoff implementations the kernels know; a subclass overriding payoff()
-# is applied in torch instead
-_PAYOFF_IMPL = (EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, BestOf)
+# payoff implementations the kernels evaluate themselves
+_IN_KERNEL = (EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, BestOf)

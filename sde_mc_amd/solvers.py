@@ -12,6 +12,7 @@ Results are computed on the current HIP device and returned on
 ``self.device`` ('cpu' by default, as in the reference).  There is no CPU
 path: without a HIP device every simulation raises HipUnavailableError.
 """
+import math
 from abc import ABC
 
 import numpy as np
@@ -33,10 +34,8 @@ class SdeSolver(ABC):
         self.num_steps = num_steps
         self.device = device
         self.has_jumps = bool(torch.as_tensor(self.sde.jump_rate()).any())
-        if len(self.sde.corr_matrix) > 1:
-            self.lower_cholesky = torch.linalg.cholesky(torch.as_tensor(self.sde.corr_matrix))
-        else:
-            self.lower_cholesky = torch.tensor([[1.]])
+        corr = torch.as_tensor(self.sde.corr_matrix)
+        self.lower_cholesky = torch.linalg.cholesky(corr) if len(corr) > 1 else torch.ones(1, 1)
         torch.manual_seed(seed)  # the reference seeds torch here (solvers.py:37)
         self.seed = int(seed)
         self.path_counter = 0
@@ -46,15 +45,15 @@ class SdeSolver(ABC):
 
     # -- reference helpers ------------------------------------------------
     def sample_corr_normals(self, size, h, corr=True):
-        """Scaled, correlated normals L (z sqrt(h)) (solvers.py:51-56)."""
-        normals = torch.randn(size=size) * torch.sqrt(torch.as_tensor(h))
-        if corr:
-            return torch.matmul(self.lower_cholesky, normals).squeeze(-1)
-        return normals.squeeze(-1)
+        """N(0, h) increments of shape `size` (trailing axis of length 1), mixed
+        by the Cholesky factor when `corr` (solvers.py:51-56); host torch RNG."""
+        z = math.sqrt(float(h)) * torch.randn(size)
+        mixed = self.lower_cholesky @ z if corr else z
+        return mixed.squeeze(-1)
 
     def init_storage(self, bs, steps, low_storage=False):
-        paths = torch.zeros(size=(bs, steps + 1, self.sde.dim))
-        return paths
+        """Path storage (bs, steps + 1, dim) for callers that step by hand."""
+        return torch.zeros(bs, steps + 1, self.sde.dim)
 
     # -- kernel plumbing --------------------------------------------------
     def _grid_code(self):
