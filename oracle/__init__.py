@@ -109,10 +109,12 @@ def ln_unit(w):
 
 
 def poisson_thresholds(mu):
+    """(thresholds T_1..T_16, reciprocal interval widths [16], counts above 16
+    negligible) of the per-step Poisson count for mean mu."""
     T = (ctypes.c_uint32 * 16)()
-    inv = ctypes.c_double()
-    ok = load().oracle_poisson_thresholds(float(mu), T, ctypes.byref(inv))
-    return [int(v) for v in T], inv.value, bool(ok)
+    inv = (ctypes.c_double * 16)()
+    ok = load().oracle_poisson_thresholds(float(mu), T, inv)
+    return [int(v) for v in T], [float(v) for v in inv], bool(ok)
 
 
 def step_jumps(job, path, n):

@@ -95,8 +95,10 @@ ORACLE_API float oracle_ln_unit(uint32_t w) {
 }
 
 /* P(N >= k) 2^32, k = 1..16, of N ~ Poisson(mu), rounded to integers; the
- * tail sums run from the top.  Returns whether P(N > 16) < 2^-33. */
-static int poisson_thresholds(double mu, uint32_t T16[16], double* inv) {
+ * tail sums run from the top.  inv[c-1] = 1 / (T_c - T_{c+1}) (T_17 = 0) is
+ * the reciprocal width of count c's interval, 0 for an empty interval.
+ * Returns whether P(N > 16) < 2^-33. */
+static int poisson_thresholds(double mu, uint32_t T16[16], double inv[16]) {
   double p[64], tail[65];
   p[0] = exp(-mu);
   for (int i = 1; i < 64; ++i) p[i] = p[i - 1] * mu / (double)i;
@@ -106,7 +108,10 @@ static int poisson_thresholds(double mu, uint32_t T16[16], double* inv) {
     double v = tail[k] * 4294967296.0;
     T16[k - 1] = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(v + 0.5);
   }
-  *inv = T16[0] ? 1.0 / (double)T16[0] : 0.0;
+  for (int k = 0; k < 16; ++k) {
+    uint32_t width = T16[k] - (k < 15 ? T16[k + 1] : 0u);
+    inv[k] = width ? 1.0 / (double)width : 0.0;
+  }
   return tail[17] * 4294967296.0 < 0.5;
 }
 
@@ -134,7 +139,7 @@ typedef struct {
   float inv_rate, n_over_T;
   int pois;
   uint32_t pT[16];
-  double pinv;
+  double pinv[16];
 } model_t;
 
 static void model_init(model_t* m, const sdemc_job* job) {
@@ -151,12 +156,12 @@ static void model_init(model_t* m, const sdemc_job* job) {
   m->n_over_T = (float)((double)job->steps / job->T);
   m->pois = 0;
   memset(m->pT, 0, sizeof(m->pT));
-  m->pinv = 0.0;
+  memset(m->pinv, 0, sizeof(m->pinv));
   /* per-step Poisson counts: Levy marks on the uniform grid */
   if (job->model.jump_law == SDEMC_JUMP_LEVY && job->grid == SDEMC_GRID_UNIFORM &&
       job->model.rate > 0.0) {
     double mu = job->model.rate * job->T / (double)job->steps;
-    m->pois = poisson_thresholds(mu, m->pT, &m->pinv);
+    m->pois = poisson_thresholds(mu, m->pT, m->pinv);
   }
 }
 
@@ -305,9 +310,11 @@ static int32_t step_of(float tau, float n_over_T, int32_t n) {
   return q < (float)n ? (int32_t)q : 0x7fffffff;
 }
 
-/* marks of step n under per-step Poisson counts: count = #{k: w < T_k};
- * mark 0 from the residual (w + 1/2) / T_1, mark k >= 1 from word (k-1) % 4
- * of mark-stream block (n, 3 + 16 ((k-1) / 4)); returns the number of marks */
+/* marks of step n under per-step Poisson counts: count c = #{k: w < T_k};
+ * mark 0 from the offset of w in its count interval, ((w - T_{c+1}) + 1/2)
+ * times the interval's reciprocal width (an fp64 product), mark k >= 1 from
+ * word (k-1) % 4 of mark-stream block (n, 3 + 16 ((k-1) / 4)); returns the
+ * number of marks */
 static int poisson_marks(const model_t* m, src_t* s, int32_t n, double* marks, int cap) {
   uint32_t pw[4];
   philox_block((uint32_t)n >> 2, 2, s->path, s->seed, pw);
@@ -317,8 +324,8 @@ static int poisson_marks(const model_t* m, src_t* s, int32_t n, double* marks, i
    * first mark's uniform, independent of c */
   int c = 1;
   while (c < 16 && w < m->pT[c]) ++c;
-  uint32_t lo = c < 16 ? m->pT[c] : 0u, hi = m->pT[c - 1];
-  double r = ((double)(w - lo) + 0.5) / (double)(hi - lo);
+  uint32_t lo = c < 16 ? m->pT[c] : 0u;
+  double r = ((double)(w - lo) + 0.5) * m->pinv[c - 1];
   float y = (float)r;
   if (y > 0x1.fffffep-1f) y = 0x1.fffffep-1f;
   for (int k = 0; k < c; ++k) {

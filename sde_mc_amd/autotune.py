@@ -16,11 +16,10 @@ import torch
 from . import _lib as L
 from .batch import BatchRunner, BatchTune, chunks_of, job_class
 
-# A CU has 4 SIMDs and the tile kernels fit at most 7 wavefronts per SIMD
-# (their scalar-register use), so the shapes swept are workgroups of 2, 4 or
-# 8 wavefronts with 3..7 resident wavefronts per SIMD in all, and 1 or 2
-# chunks per tile.
-WAVES_PER_SIMD = (3, 4, 5, 6, 7)
+# A CU has 4 SIMDs and the tile kernels fit 7 or 8 wavefronts per SIMD, so
+# the shapes swept are workgroups of 2, 4 or 8 wavefronts with 3..8 resident
+# wavefronts per SIMD in all, and 1 or 2 chunks per tile.
+WAVES_PER_SIMD = (3, 4, 5, 6, 7, 8)
 CANDIDATES = [BatchTune(w, 4 * s // w, t) for w in (2, 4, 8) for s in WAVES_PER_SIMD
               for t in (1, 2) if (4 * s) % w == 0]
 

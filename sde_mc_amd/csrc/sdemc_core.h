@@ -46,6 +46,14 @@ struct U4 {
 
 __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                       uint32_t c3, uint32_t k0, uint32_t k1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the key is wave-uniform (one seed per job): recompute its schedule per
+  // block on the scalar unit rather than holding 20 round keys in scalar
+  // registers for the whole kernel
+  k0 = __builtin_amdgcn_readfirstlane(k0);
+  k1 = __builtin_amdgcn_readfirstlane(k1);
+  asm volatile("" : "+s"(k0), "+s"(k1));
+#endif
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -186,13 +194,14 @@ struct KJob {
   // offset of w within its count's interval (see poisson_count)
   int32_t pois_mode;
   uint32_t pois_T[16];
-  double pois_inv;
+  double pois_inv[16];  // reciprocal width of count c's interval at [c - 1]
 };
 
 // Thresholds of the per-step jump count: pois_T[k] = round(P(N >= k+1) 2^32)
 // for N ~ Poisson(mu), mu = rate * T / steps; the tail sums run from the top
-// so small tails keep full relative precision.  Returns whether counts above
-// 16 are negligible (P(N > 16) < 2^-33).
+// so small tails keep full relative precision.  inv[c-1] = 1 / (T[c-1] - T[c])
+// (T[16] = 0), 0 for an empty interval.  Returns whether counts above 16 are
+// negligible (P(N > 16) < 2^-33).
 inline bool poisson_thresholds(double mu, uint32_t* T16, double* inv) {
 #pragma clang fp contract(off)
   double p[64];
@@ -205,7 +214,10 @@ inline bool poisson_thresholds(double mu, uint32_t* T16, double* inv) {
     const double v = tail[k] * 4294967296.0;
     T16[k - 1] = v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)(v + 0.5);
   }
-  *inv = T16[0] ? 1.0 / (double)T16[0] : 0.0;
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t width = T16[k] - (k < 15 ? T16[k + 1] : 0u);
+    inv[k] = width ? 1.0 / (double)width : 0.0;
+  }
   return tail[17] * 4294967296.0 < 0.5;
 }
 
@@ -283,11 +295,13 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.magic = KJOB_MAGIC;
   k.class_key = class_key_of(m.noise, m.dim, j.grid, j.coarse_factor > 0);
   k.pois_mode = 0;
-  for (int q = 0; q < 16; ++q) k.pois_T[q] = 0;
-  k.pois_inv = 0.0;
+  for (int q = 0; q < 16; ++q) {
+    k.pois_T[q] = 0;
+    k.pois_inv[q] = 0.0;
+  }
   if (m.jump_law == SDEMC_JUMP_LEVY && j.grid == SDEMC_GRID_UNIFORM && m.rate > 0.0) {
     const double mu = m.rate * j.T / (double)j.steps;
-    k.pois_mode = poisson_thresholds(mu, k.pois_T, &k.pois_inv) ? 1 : 0;
+    k.pois_mode = poisson_thresholds(mu, k.pois_T, k.pois_inv) ? 1 : 0;
   }
 }
 
@@ -495,12 +509,32 @@ __device__ __forceinline__ int32_t step_of(float tau, float n_over_T, int32_t n)
 // pois_T[c] <= w < pois_T[c-1] (pois_T[16] = 0), and given c the offset of
 // w in that interval is uniform and independent of c, so it yields the
 // first mark's uniform: y = (w - lo + 1/2) / (hi - lo), fp64, one rounding.
+// Count c of a step with w < T[0]: T[c] <= w < T[c-1]; y is the first mark's
+// uniform, the offset of w inside that interval times the interval's
+// reciprocal width.  Counts 1 and 2 use values the compiler keeps in
+// registers; larger counts (rare for every wavefront) scan the rest of the
+// table in a loop kept rolled, so its entries are read from memory when
+// needed instead of occupying scalar registers for the whole kernel.
 __device__ __forceinline__ int poisson_count(const KJob& j, uint32_t w, float& y) {
-  int c = 1;
-  while (c < 16 && w < j.pois_T[c]) ++c;
-  const uint32_t lo = c < 16 ? j.pois_T[c] : 0u;
-  const uint32_t hi = j.pois_T[c - 1];
-  const double r = ((double)(w - lo) + 0.5) / (double)(hi - lo);
+  int c;
+  uint32_t lo;
+  double inv;
+  if (w >= j.pois_T[1]) {
+    c = 1;
+    lo = j.pois_T[1];
+    inv = j.pois_inv[0];
+  } else if (w >= j.pois_T[2]) {
+    c = 2;
+    lo = j.pois_T[2];
+    inv = j.pois_inv[1];
+  } else {
+    c = 3;
+#pragma nounroll
+    while (c < 16 && w < j.pois_T[c]) ++c;
+    lo = c < 16 ? j.pois_T[c] : 0u;
+    inv = j.pois_inv[c - 1];
+  }
+  const double r = ((double)(w - lo) + 0.5) * inv;
   y = fminf((float)r, 0x1.fffffep-1f);
   return c;
 }
@@ -738,38 +772,66 @@ __device__ __forceinline__ void fixed_row(const float* row, int64_t P, float (&d
   }
 }
 
+// One uniform-grid step fed from fixed inputs: the Euler update with the
+// step's increments, then its `cnt` jumps with marks read in CSR order.
+template <int D, int FAM>
+__device__ __forceinline__ void fixed_step(const KJob& j, const sdemc_fixed_in& in, float (&x)[D],
+                                           const float (&dW)[D], const float (&dB)[D], int cnt,
+                                           int64_t& off, int& bad) {
+  const KModel& f = j.m;
+  float xo[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) xo[i] = x[i];
+  euler<D, FAM>(f, x, j.h, dW, dB);
+  for (int k = 0; k < cnt; ++k) {
+    const bool ok = off >= 0 && off < in.n_marks;
+    const float J = ok ? in.jump_size_dev[off] : 0.0f;
+    bad |= !ok;
+    ++off;
+    if (f.exact) {
+      float y[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) y[i] = x[i];
+      add_jump(f, x, y, J);
+    } else {
+      add_jump(f, x, xo, J);
+    }
+  }
+}
+
 // Fixed increments on the uniform grid: dw [N][K][P] already scaled and
-// correlated, counts [N][P], marks in CSR order per path.
+// correlated, counts [N][P], marks in CSR order per path.  The inputs of G
+// consecutive steps are loaded before the G dependent updates run, so a
+// wavefront keeps G rows of loads in flight instead of one.
 template <int D, int FAM>
 __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int64_t p, int64_t P,
                                    float (&x)[D], int& bad) {
   constexpr int KF = NKF<D, FAM>::K;
+  constexpr int G = D <= 2 ? 8 : 4;
   const KModel& f = j.m;
   const int32_t N = j.steps;
   int64_t off = in.jump_off_dev ? in.jump_off_dev[p] : 0;
 #pragma unroll
   for (int i = 0; i < D; ++i) x[i] = f.x0[i];
-  for (int32_t n = 0; n < N; ++n) {
-    float dW[D], dB[D], xo[D];
-    fixed_row<D, FAM>(in.dw_dev + (int64_t)n * KF * P + p, P, dW, dB);
+  const float* dwp = in.dw_dev + p;
+  const uint8_t* cntp = in.counts_dev ? in.counts_dev + p : nullptr;
+  int32_t n = 0;
+  for (; n + G <= N; n += G) {
+    float dW[G][D], dB[G][D];
+    int cnt[G];
 #pragma unroll
-    for (int i = 0; i < D; ++i) xo[i] = x[i];
-    euler<D, FAM>(f, x, j.h, dW, dB);
-    const int cnt = in.counts_dev ? (int)in.counts_dev[(int64_t)n * P + p] : 0;
-    for (int k = 0; k < cnt; ++k) {
-      const bool ok = off >= 0 && off < in.n_marks;
-      const float J = ok ? in.jump_size_dev[off] : 0.0f;
-      bad |= !ok;
-      ++off;
-      if (f.exact) {
-        float y[D];
-#pragma unroll
-        for (int i = 0; i < D; ++i) y[i] = x[i];
-        add_jump(f, x, y, J);
-      } else {
-        add_jump(f, x, xo, J);
-      }
+    for (int g = 0; g < G; ++g) {
+      fixed_row<D, FAM>(dwp + (int64_t)(n + g) * KF * P, P, dW[g], dB[g]);
+      cnt[g] = cntp ? (int)cntp[(int64_t)(n + g) * P] : 0;
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g) fixed_step<D, FAM>(j, in, x, dW[g], dB[g], cnt[g], off, bad);
+  }
+  for (; n < N; ++n) {
+    float dW[D], dB[D];
+    fixed_row<D, FAM>(dwp + (int64_t)n * KF * P, P, dW, dB);
+    const int cnt = cntp ? (int)cntp[(int64_t)n * P] : 0;
+    fixed_step<D, FAM>(j, in, x, dW, dB, cnt, off, bad);
   }
 }
 

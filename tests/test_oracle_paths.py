@@ -303,7 +303,8 @@ def test_poisson_thresholds_match_scipy():
         assert ok == (stats.poisson.sf(16, mu) * 2 ** 32 < 0.5)
         for k in range(1, 17):
             assert abs(T[k - 1] - stats.poisson.sf(k - 1, mu) * 2 ** 32) <= 0.51
-        assert inv == 1.0 / T[0]
+        widths = [T[k] - (T[k + 1] if k < 15 else 0) for k in range(16)]
+        assert inv == [1.0 / w if w else 0.0 for w in widths]
 
 
 def test_per_step_counts_are_poisson_and_marks_follow_the_levy_law():
