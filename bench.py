@@ -64,6 +64,14 @@ def cpu_baseline(reqs, budget_s=12.0, cap_paths=1 << 12):
 
 MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2, N(-0.1, 0.15^2), T=1
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# each scaled by the counter's reading on a 1 GiB device copy (fetch x2.0,
+# write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
+# runs.  Served mix round (k_batch group kernels, 128 requests, seed 0) and
+# the fixed-increment kernel (2^21 x 500).
+TRAFFIC_MIX_ROUND = 8.313e7
+TRAFFIC_FIXED = 6.102e9
+
 
 def _time_events(fn, reps):
     """Mean device time (ms) of fn() over reps, bracketed by events on the
@@ -117,6 +125,7 @@ def fixed_increment_leg(dev, reps, P=1 << 21, N=500, seed=0):
             "bytes_per_launch": nbytes, "bytes_per_path_step": nbytes / (P * N),
             "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
                          "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
+                         "traffic": TRAFFIC_FIXED if P == 1 << 21 and N == 500 else None,
                          "kernel": "k_fixed<1,merton,uniform> + k_finalize_one"},
             "price": mean, "se": se, "closed_form": MERTON_CLOSED_FORM}
 
@@ -361,7 +370,8 @@ def main():
         "latency": latency,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
                      "unit": "Top/s (VALU lane-ops, algorithmic)",
-                     "frac": achieved / PEAK_VALU_OPS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_OPS,
+                     "traffic": TRAFFIC_MIX_ROUND if (args.requests, args.seed) == (128, 0) else None,
                      "kernel": "k_batch (persistent tile kernel)", "kernel_ms": kern_ms},
         "cpu_baseline": cpu,
     }
