@@ -321,14 +321,18 @@ def main():
     if not args.no_serve:
         from sde_mc_amd.serve import PricingServer, latency_summary, serve_plain
         latency = {}
-        for rate in (50, 200, 1000):
-            arr = W.request_mix(args.requests, seed=args.seed + rate, rate=rate)
+        # 50, 200, 1000 req/s: the workload's rates; 1500 req/s offers more
+        # work than the plain per-request form can finish (~2.8e11
+        # path-steps/s) but less than the batched rounds can
+        for rate in (50, 200, 1000, 1500):
+            n_req = args.requests * (4 if rate == 1500 else 1)   # long enough to queue up
+            arr = W.request_mix(n_req, seed=args.seed + rate, rate=rate)
             srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20, tune=tune)
             res = srv.serve(arr)
             served = latency_summary(res)
             served["rounds"] = srv.rounds
             served["peak_partial_bytes"] = srv.peak_used * 20
-            arr2 = W.request_mix(args.requests, seed=args.seed + rate, rate=rate)
+            arr2 = W.request_mix(n_req, seed=args.seed + rate, rate=rate)
             plain_res = serve_plain(arr2)
             same = all(np.array_equal(np.array(res[i].stats[:3]), np.array(plain_res[i].stats[:3]))
                        for i in res)

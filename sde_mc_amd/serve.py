@@ -3,13 +3,15 @@
 Requests (model, strike, maturity, steps, paths, seed) arrive over time.  A
 request is admitted when a job slot is free and its chunk partials fit in
 the HBM budget; its paths are then priced in rounds.  Each round is one
-batched persistent launch (sde_mc_amd.batch) over a fair share of every
-admitted request's remaining 64-path chunks, so requests that arrive while
-others are on the GPU join the next round and their work shares the device
-with everything else in flight.  Two rounds are kept in flight so the host
-plans round r+1 while round r runs.  A request's partial sums stay on the
-device until its last chunk is done; it is then finalised in-kernel and its
-(sum, sum of squares, n) is read back.
+batched persistent launch (sde_mc_amd.batch) over chunks of the admitted
+requests -- by default the requests with the least work left first, so
+small requests finish within a round or two of their arrival while large
+ones keep the device busy -- so requests that arrive while others are on the
+GPU join the next round and their work shares the device with everything
+else in flight.  Two rounds are kept in flight so the host plans round r+1
+while round r runs.  A request's partial sums stay on the device until its
+last chunk is done; it is then finalised in-kernel and its (sum, sum of
+squares, n) is read back.
 
 Results do not depend on the mix: each request's paths are keyed by its own
 seed and path index, and its reduction order is fixed (see batch.py), so a
@@ -86,20 +88,40 @@ class _Active:
 
 
 class PricingServer:
-    """Admission by slots and HBM budget, fair-share rounds of batched launches."""
+    """Admission by slots and HBM budget, rounds of batched launches.
 
-    def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=2.0, tune=None,
-                 grid=L.GRID_UNIFORM, job_builder=None):
+    policy 'srpt' (default) fills each round with the requests that have the
+    least work left, each taking all its remaining chunks while the round's
+    budget lasts; 'fair' gives the oldest request half the budget, the next
+    half of the rest, and so on.  A round that holds a single request from its
+    first chunk to its last runs that request's plain kernel instead (the
+    same bits, less launch overhead)."""
+
+    def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=4.0, tune=None,
+                 grid=L.GRID_UNIFORM, job_builder=None, policy='srpt', plain_single=True,
+                 tile_chunks=1):
+        if policy not in ('srpt', 'fair'):
+            raise ValueError("policy must be 'srpt' or 'fair'")
+        self.policy = policy
+        self.plain_single = plain_single
         self.pool = ChunkPool(hbm_budget_bytes // PARTIAL_BYTES)
+        # rounds are a few ms long, so tiles stay one chunk (a long tile at the
+        # end of a round leaves the rest of the device idle)
+        t = tune or BatchTune()
+        t = BatchTune(t.waves_per_block, t.blocks_per_cu, tile_chunks, t.concurrent_groups)
         self.runner = BatchRunner(max_jobs=max_slots, max_segments=2 * max_slots,
-                                  partial_chunks=self.pool.capacity, tune=tune or BatchTune(),
-                                  n_buffers=2)
+                                  partial_chunks=self.pool.capacity, tune=t, n_buffers=2)
         self.max_slots = max_slots
         self.free_slots = list(range(max_slots))[::-1]
         self.queue = deque()
         self.active = {}
         self.round_s = round_ms * 1e-3
         self.cost_rate = 1.2e12     # cost units (VALU slots) per second, refined per round
+        # a round never holds fewer chunks than fill the device twice over
+        # (unless less work is left): smaller rounds leave CUs idle
+        t = self.runner.tune
+        cus = torch.cuda.get_device_properties(self.runner.device).multi_processor_count
+        self.fill_chunks = 2 * cus * t.waves_per_block * t.blocks_per_cu * max(1, t.tile_chunks)
         self.grid = grid
         self.job_builder = job_builder or (lambda r: request_job(r, grid))
         self.in_flight = deque()
@@ -146,30 +168,47 @@ class PricingServer:
 
     # -- rounds ------------------------------------------------------------------
     def _plan(self):
-        """Split the round's cost budget over the admitted requests with chunks
-        left, oldest first: each takes at most half of what is left (at least
-        one chunk), so the oldest request keeps most of the device while every
-        newcomer still gets work into the round.  A request whose last chunk
-        is in the round is finalised in it."""
+        """Chunks of each admitted request for the next round (see the class
+        docstring for the policies).  A round below `fill_chunks` is topped
+        up, in the same order, so it still fills the device.  A request whose
+        last chunk is in the round is finalised in it."""
         todo = [a for a in self.active.values() if a.next < a.nc]
         if not todo:
             return None
-        todo.sort(key=lambda a: a.admitted)
         budget = self.cost_rate * self.round_s
         take = {a.slot: 0 for a in todo}
         left = budget
-        for _ in range(2):
+        if self.policy == 'srpt':
+            todo.sort(key=lambda a: ((a.nc - a.next) * a.cost, a.admitted))
             for a in todo:
-                rem = a.nc - a.next - take[a.slot]
-                if rem <= 0:
-                    continue
-                if left <= 0 and take[a.slot] > 0:
-                    continue
-                k = min(rem, max(1, int(max(0.5 * left, 0.0) / a.cost)))
-                take[a.slot] += k
+                if left <= 0:
+                    break
+                rem = a.nc - a.next
+                k = min(rem, max(1, int(left / a.cost)))
+                take[a.slot] = k
                 left -= k * a.cost
-            if left <= 0:
+        else:
+            todo.sort(key=lambda a: a.admitted)
+            for _ in range(2):
+                for a in todo:
+                    rem = a.nc - a.next - take[a.slot]
+                    if rem <= 0:
+                        continue
+                    if left <= 0 and take[a.slot] > 0:
+                        continue
+                    k = min(rem, max(1, int(max(0.5 * left, 0.0) / a.cost)))
+                    take[a.slot] += k
+                    left -= k * a.cost
+                if left <= 0:
+                    break
+        short = self.fill_chunks - sum(take.values())
+        for a in todo:
+            if short <= 0:
                 break
+            k = min(a.nc - a.next - take[a.slot], short)
+            if k > 0:
+                take[a.slot] += k
+                short -= k
         slots, firsts, counts, costs, final = [], [], [], [], []
         for a in todo:
             k = take[a.slot]
@@ -186,35 +225,64 @@ class PricingServer:
 
     def _launch(self, plan):
         slots, firsts, counts, costs, final = plan
-        buf = next(b for b in self.runner.bufs if b not in [f[0] for f in self.in_flight])
-        rp = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks,
-                           self.runner.slot_class[np.asarray(slots, dtype=np.int64)])
-        self.runner.stage_round(buf, rp, final)
         start = torch.cuda.Event(enable_timing=True)
-        start.record()
-        self.runner.launch(buf, rp.groups)
-        self.runner.read_stats(buf, final)
         cost = float(np.dot(counts, costs))
-        self.in_flight.append((buf, final, start, cost))
+        full = sum(counts) >= self.fill_chunks
+        if (self.plain_single and len(slots) == 1 and firsts[0] == 0
+                and final == slots):
+            # one whole request: its plain kernel gives the same bits
+            a = self.active[slots[0]]
+            start.record()
+            stats = E.simulate(a.job, self.runner.device)
+            host = self._host_stats()
+            host.copy_(stats, non_blocking=True)
+            done = torch.cuda.Event(enable_timing=True)
+            done.record()
+            self.in_flight.append(('plain', (host, done, slots[0]), final, start,
+                                   cost if full else 0.0))
+        else:
+            buf = next(b for b in self.runner.bufs
+                       if b not in [f[1] for f in self.in_flight if f[0] == 'batch'])
+            rp = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks,
+                               self.runner.slot_class[np.asarray(slots, dtype=np.int64)])
+            self.runner.stage_round(buf, rp, final)
+            start.record()
+            self.runner.launch(buf, rp.groups)
+            self.runner.read_stats(buf, final)
+            self.in_flight.append(('batch', buf, final, start, cost if full else 0.0))
         self.rounds += 1
+
+    def _host_stats(self):
+        """A pinned 4-double buffer for a plain round's read-back (a small ring:
+        at most two rounds are in flight)."""
+        ring = getattr(self, '_ring', None)
+        if ring is None:
+            ring = self._ring = [torch.zeros(4, dtype=torch.float64, pin_memory=True)
+                                 for _ in range(4)]
+            self._ring_i = 0
+        self._ring_i = (self._ring_i + 1) % len(ring)
+        return ring[self._ring_i]
 
     def _complete(self, block):
         """Retire finished rounds (the oldest first); block on it if asked."""
         while self.in_flight:
-            buf, final, start, cost = self.in_flight[0]
-            if not block and not buf.done.query():
+            kind, what, final, start, cost = self.in_flight[0]
+            done = what[1] if kind == 'plain' else what.done
+            if not block and not done.query():
                 return
-            buf.done.synchronize()
+            done.synchronize()
             block = False
             self.in_flight.popleft()
-            ms = start.elapsed_time(buf.done)
-            if ms > 0.05:  # refine the throughput estimate that sizes rounds
+            ms = start.elapsed_time(done)
+            if cost > 0 and ms > 0.05:  # rounds that filled the device refine the rate
                 self.cost_rate = 0.7 * self.cost_rate + 0.3 * (cost / (ms * 1e-3))
             now = self._now()
-            st = buf.h_stats.numpy()
             for s in final:
                 a = self.active.pop(s)
-                s4 = st[4 * s:4 * s + 4].copy()
+                if kind == 'plain':
+                    s4 = what[0].numpy().copy()
+                else:
+                    s4 = what.h_stats.numpy()[4 * s:4 * s + 4].copy()
                 err = None
                 try:
                     E.check_flags(s4[3], 'request %d' % a.req.rid)
