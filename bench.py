@@ -242,8 +242,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the fixed-increment, equal-CI and MLMC legs")
     ap.add_argument("--tune", default=None,
-                    help="W,B,T launch shape (waves/block, blocks/CU, chunks/tile) instead of "
-                         "autotuning")
+                    help="W,B,T[,G[,U]] launch shape (waves/block, blocks/CU, chunks/tile, "
+                         "concurrent groups, step unroll) instead of autotuning")
     ap.add_argument("--fixed-only", action="store_true",
                     help="run only the fixed-increment leg and print its record")
     ap.add_argument("--probe-copy-mib", type=int, default=0,
@@ -368,7 +368,8 @@ def main():
                    "tune": {"waves_per_block": tune.waves_per_block,
                             "blocks_per_cu": tune.blocks_per_cu,
                             "tile_chunks": tune.tile_chunks,
-                            "concurrent_groups": tune.concurrent_groups}},
+                            "concurrent_groups": tune.concurrent_groups,
+                            "step_unroll": tune.step_unroll}},
         "plain_value": plain,
         "speedup_vs_plain": (batched / plain) if plain else None,
         "latency": latency,

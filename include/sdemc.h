@@ -298,7 +298,8 @@ typedef struct sdemc_batch_tune {
                                 stream; 1: forked onto side streams (default
                                 priority) and joined back, so a group's tail
                                 overlaps the next group                        */
-  int32_t reserved;
+  int32_t step_unroll;       /* 4-step groups per loop iteration of the uniform
+                                single-level path: 0 or 1 (one), 2 (two)       */
 } sdemc_batch_tune;
 
 /* One batched launch: zero the tile counters, run the group kernels,

@@ -83,7 +83,7 @@ class BatchDev(ctypes.Structure):
 
 class BatchTune(ctypes.Structure):
     _fields_ = [("waves_per_block", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("concurrent_groups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("concurrent_groups", ctypes.c_int32), ("step_unroll", ctypes.c_int32)]
 
 
 class SdemcError(RuntimeError):

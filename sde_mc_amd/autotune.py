@@ -1,8 +1,8 @@
 """Autotuner of the batched persistent launch.
 
 Sweeps the launch shape -- wavefronts per workgroup (so paths per workgroup
-= 64 x wavefronts), resident workgroups per CU and chunks per tile -- on a
-calibration batch, timing the tile kernel with device events, and keeps the
+= 64 x wavefronts), resident workgroups per CU, chunks per tile and the step
+unroll -- on a calibration batch, timing the tile kernel with device events, and keeps the
 fastest shape per (device, set of kernel classes in the batch).  Results are cached in a JSON
 file next to the library so later processes start tuned.
 """
@@ -17,11 +17,12 @@ from . import _lib as L
 from .batch import BatchRunner, BatchTune, chunks_of, job_class
 
 # A CU has 4 SIMDs and the tile kernels fit 7 or 8 wavefronts per SIMD, so
-# the shapes swept are workgroups of 2, 4 or 8 wavefronts with 3..8 resident
-# wavefronts per SIMD in all, and 1 or 2 chunks per tile.
+# the shapes swept are workgroups of 2, 4 or 8 wavefronts (128, 256 or 512
+# paths) with 3..8 resident wavefronts per SIMD in all, 1 or 2 chunks per
+# tile, and a step unroll of 1 or 2 four-step groups per loop iteration.
 WAVES_PER_SIMD = (3, 4, 5, 6, 7, 8)
-CANDIDATES = [BatchTune(w, 4 * s // w, t) for w in (2, 4, 8) for s in WAVES_PER_SIMD
-              for t in (1, 2) if (4 * s) % w == 0]
+CANDIDATES = [BatchTune(w, 4 * s // w, t, 1, u) for w in (2, 4, 8) for s in WAVES_PER_SIMD
+              for t in (1, 2) for u in (1, 2) if (4 * s) % w == 0]
 
 
 def cache_path():

@@ -71,28 +71,35 @@ def job_class(job):
 
 
 class BatchTune:
-    """Launch shape of the persistent kernels: wavefronts per workgroup,
-    resident workgroups per CU, least chunks (64 paths each) per tile, and
-    whether a round's class groups run concurrently."""
+    """Launch shape of the persistent kernels: wavefronts per workgroup (paths
+    per workgroup = 64 x that), resident workgroups per CU, least chunks (64
+    paths each) per tile, whether a round's class groups run concurrently,
+    and the step unroll (4-step groups per loop iteration, 1 or 2) of the
+    uniform single-level kernels."""
 
-    def __init__(self, waves_per_block=4, blocks_per_cu=7, tile_chunks=1, concurrent_groups=1):
+    def __init__(self, waves_per_block=4, blocks_per_cu=7, tile_chunks=1, concurrent_groups=1,
+                 step_unroll=1):
         self.waves_per_block = int(waves_per_block)
         self.blocks_per_cu = int(blocks_per_cu)
         self.tile_chunks = int(tile_chunks)
         self.concurrent_groups = int(concurrent_groups)
+        self.step_unroll = int(step_unroll)
 
     def c(self):
         t = L.BatchTune()
         t.waves_per_block = self.waves_per_block
         t.blocks_per_cu = self.blocks_per_cu
         t.concurrent_groups = self.concurrent_groups
+        t.step_unroll = self.step_unroll
         return t
 
     def key(self):
-        return (self.waves_per_block, self.blocks_per_cu, self.tile_chunks, self.concurrent_groups)
+        return (self.waves_per_block, self.blocks_per_cu, self.tile_chunks, self.concurrent_groups,
+                self.step_unroll)
 
     def __repr__(self):
-        return 'BatchTune(wpb=%d, bpc=%d, tile_chunks=%d, concurrent_groups=%d)' % self.key()
+        return ('BatchTune(wpb=%d, bpc=%d, tile_chunks=%d, concurrent_groups=%d, step_unroll=%d)'
+                % self.key())
 
 
 class RoundPlan:
@@ -306,8 +313,7 @@ class BatchRunner:
         groups = tuple(int(g) for g in groups) or (L.CLASS_GENERIC,)
         stream = E.stream_ptr()
         if self.use_graph:
-            key = (id(buf), groups, tune.waves_per_block, tune.blocks_per_cu,
-                   tune.concurrent_groups, self.partials.data_ptr())
+            key = (id(buf), groups, tune.key(), self.partials.data_ptr())
             plan = self._plans.get(key)
             if plan is None:
                 h = ctypes.c_void_p()

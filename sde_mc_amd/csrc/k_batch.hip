@@ -17,7 +17,7 @@ int validate_job(const sdemc_job* job);
 
 // One 64-path chunk of one job, RNG mode, no per-path outputs.  The result
 // equals the plain kernel's chunk: same Philox streams, same butterfly.
-template <int D, int FAM, int GRID, bool COUPLED>
+template <int D, int FAM, int GRID, bool COUPLED, int U = 1>
 __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, double* part,
                                            int32_t* cf) {
   const int lane = threadIdx.x & (WAVE - 1);
@@ -36,7 +36,7 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
   if (!COUPLED) {
     float x[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      path_uniform_rng<D, FAM, false>(job, c, valid, x, st);
+      path_uniform_rng<D, FAM, false, U>(job, c, valid, x, st);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);
@@ -72,7 +72,7 @@ inline int32_t job_class_of(int noise, int dim, int grid, bool coupled) {
 
 #define SDEMC_CASE(NOISE, D, FAM, GRID, C)                              \
   case ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + C:                      \
-    chunk_body<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);           \
+    chunk_body<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);        \
     return true;
 
 #define SDEMC_CASES_DG(NOISE, D, FAM, GRID) \
@@ -82,7 +82,7 @@ inline int32_t job_class_of(int noise, int dim, int grid, bool coupled) {
 // One chunk of a job in kernel class CLS: the dedicated classes run exactly
 // one (family, grid, level type) -- the code of the plain kernel for it --
 // and refuse anything else; the generic class runs every supported job.
-template <int CLS>
+template <int CLS, int U>
 __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double* part,
                                           int32_t* cf) {
   if (CLS < SDEMC_CLASS_GENERIC) {
@@ -92,7 +92,7 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
     constexpr int D = FAMI == 0 ? 1 : 2;
     constexpr int FAM = FAMI == 0 ? FAM_MERTON : FAM_LEVY;
     if (job.class_key != ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + (COUPLED ? 1 : 0)) return false;
-    chunk_body<D, FAM, GRID, COUPLED>(job, chunk, part, cf);
+    chunk_body<D, FAM, GRID, COUPLED, U>(job, chunk, part, cf);
     return true;
   } else {
     switch (job.class_key) {
@@ -125,8 +125,9 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
 #undef SDEMC_CASES_DG
 
 // Persistent tile kernel of one class group.  hdr = {first segment,
-// segments, tiles}; tile indices come from this group's counter.
-template <int CLS>
+// segments, tiles}; tile indices come from this group's counter.  U is the
+// step unroll of the uniform single-level path (classes 0 and 4).
+template <int CLS, int U>
 __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
                                                const sdemc_segment* __restrict__ segs,
                                                const int32_t* __restrict__ hdr,
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
     for (int64_t chunk = c0; chunk < c1; ++chunk) {
       const int64_t slot = job.chunk_base + chunk;
       if (slot < 0 || slot >= partial_chunks) break;
-      if (!run_chunk<CLS>(job, chunk, partials + 2 * slot, cflags + slot) && lane == 0) {
+      if (!run_chunk<CLS, U>(job, chunk, partials + 2 * slot, cflags + slot) && lane == 0) {
         // a job outside this kernel's class: poison its partial and flag it
         partials[2 * slot] = __builtin_nan("");
         partials[2 * slot + 1] = __builtin_nan("");
@@ -214,6 +215,7 @@ static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {
   const int w = tune->waves_per_block;
   if (w != 1 && w != 2 && w != 4 && w != 8) return SDEMC_ERR_ARG;
   if (tune->blocks_per_cu < 1 || tune->blocks_per_cu > 64) return SDEMC_ERR_ARG;
+  if (tune->step_unroll < 0 || tune->step_unroll > 2) return SDEMC_ERR_ARG;
   return SDEMC_OK;
 }
 
@@ -229,9 +231,9 @@ static int cu_count_cached() {
   return cus;
 }
 
-template <int CLS>
+template <int CLS, int U = 1>
 static void launch_group(const sdemc_batch_dev& b, int g, unsigned grid, int wpb, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch<CLS>, dim3(grid), dim3(WAVE * wpb), 0, s, (const KJob*)b.jobs_dev,
+  hipLaunchKernelGGL((k_batch<CLS, U>), dim3(grid), dim3(WAVE * wpb), 0, s, (const KJob*)b.jobs_dev,
                      b.segs_dev, b.seg_hdr_dev + 4 * g, b.partials_dev, b.chunk_flags_dev,
                      b.counter_dev + g, b.partial_chunks, b.max_jobs, b.max_segments);
 }
@@ -257,10 +259,10 @@ static bool fork_create(Fork& f) {
 
 static void fork_destroy(Fork& f) {
   for (int i = 0; i < SDEMC_MAX_GROUPS - 1; ++i) {
-    if (f.join[i]) hipEventDestroy(f.join[i]);
-    if (f.side[i]) hipStreamDestroy(f.side[i]);
+    if (f.join[i]) (void)hipEventDestroy(f.join[i]);
+    if (f.side[i]) (void)hipStreamDestroy(f.side[i]);
   }
-  if (f.fork) hipEventDestroy(f.fork);
+  if (f.fork) (void)hipEventDestroy(f.fork);
   memset(&f, 0, sizeof(f));
 }
 
@@ -283,14 +285,20 @@ static Fork* device_fork() {
   return &it->second;
 }
 
-static void launch_class(const sdemc_batch_dev& b, int g, unsigned grid, int wpb,
+static void launch_class(const sdemc_batch_dev& b, int g, unsigned grid, int wpb, int unroll,
                          hipStream_t s) {
   switch (b.group_class[g]) {
-    case 0: launch_group<0>(b, g, grid, wpb, s); break;
+    case 0:
+      if (unroll == 2) launch_group<0, 2>(b, g, grid, wpb, s);
+      else launch_group<0>(b, g, grid, wpb, s);
+      break;
     case 1: launch_group<1>(b, g, grid, wpb, s); break;
     case 2: launch_group<2>(b, g, grid, wpb, s); break;
     case 3: launch_group<3>(b, g, grid, wpb, s); break;
-    case 4: launch_group<4>(b, g, grid, wpb, s); break;
+    case 4:
+      if (unroll == 2) launch_group<4, 2>(b, g, grid, wpb, s);
+      else launch_group<4>(b, g, grid, wpb, s);
+      break;
     case 5: launch_group<5>(b, g, grid, wpb, s); break;
     case 6: launch_group<6>(b, g, grid, wpb, s); break;
     case 7: launch_group<7>(b, g, grid, wpb, s); break;
@@ -305,20 +313,21 @@ static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
                          Fork* fork) {
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, b.counter_dev, b.n_groups);
   const int wpb = tune.waves_per_block;
+  const int unroll = tune.step_unroll;
   const unsigned grid = (unsigned)(cu_count_cached() * tune.blocks_per_cu);
   if (fork && b.n_groups > 1) {
     if (hipEventRecord(fork->fork, s) != hipSuccess) return SDEMC_ERR_LAUNCH;
     for (int g = 1; g < b.n_groups; ++g)
       if (hipStreamWaitEvent(fork->side[g - 1], fork->fork, 0) != hipSuccess) return SDEMC_ERR_LAUNCH;
-    launch_class(b, 0, grid, wpb, s);
+    launch_class(b, 0, grid, wpb, unroll, s);
     for (int g = 1; g < b.n_groups; ++g) {
-      launch_class(b, g, grid, wpb, fork->side[g - 1]);
+      launch_class(b, g, grid, wpb, unroll, fork->side[g - 1]);
       if (hipEventRecord(fork->join[g - 1], fork->side[g - 1]) != hipSuccess) return SDEMC_ERR_LAUNCH;
     }
     for (int g = 1; g < b.n_groups; ++g)
       if (hipStreamWaitEvent(s, fork->join[g - 1], 0) != hipSuccess) return SDEMC_ERR_LAUNCH;
   } else {
-    for (int g = 0; g < b.n_groups; ++g) launch_class(b, g, grid, wpb, s);
+    for (int g = 0; g < b.n_groups; ++g) launch_class(b, g, grid, wpb, unroll, s);
   }
   hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s,
                      (const KJob*)b.jobs_dev, b.finalize_dev, b.partials_dev, b.chunk_flags_dev,
@@ -390,7 +399,7 @@ int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tu
   const bool forked = tune->concurrent_groups && b->n_groups > 1;
   if (forked && !fork_create(fork)) {
     fork_destroy(fork);
-    hipStreamDestroy(cs);
+    (void)hipStreamDestroy(cs);
     return SDEMC_ERR_LAUNCH;
   }
   sdemc_batch_plan* p = new sdemc_batch_plan();
@@ -404,10 +413,10 @@ int sdemc_batch_plan_create(const sdemc_batch_dev* b, const sdemc_batch_tune* tu
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
   if (forked) fork_destroy(fork);
-  hipStreamDestroy(cs);
+  (void)hipStreamDestroy(cs);
   if (e != hipSuccess) {
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     delete p;
     return SDEMC_ERR_LAUNCH;
   }
@@ -423,8 +432,8 @@ int sdemc_batch_plan_launch(sdemc_batch_plan* plan, void* stream) {
 
 int sdemc_batch_plan_destroy(sdemc_batch_plan* plan) {
   if (!plan) return SDEMC_OK;
-  if (plan->exec) hipGraphExecDestroy(plan->exec);
-  if (plan->graph) hipGraphDestroy(plan->graph);
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
   delete plan;
   return SDEMC_OK;
 }

@@ -657,7 +657,9 @@ struct StoreCtx {
 // four steps come from K Philox blocks generated together, so every index
 // into them is a compile-time constant.
 // ===========================================================================
-template <int D, int FAM, bool STORE>
+// U: 4-step groups per iteration of the step loop (code shape only; the
+// draws and the arithmetic are the same for every U).
+template <int D, int FAM, bool STORE, int U = 1>
 __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_valid, float (&x)[D],
                                  const StoreCtx& st) {
   constexpr int K = NK<D, FAM>::K;
@@ -679,7 +681,7 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
 #pragma unroll
     for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
   }
-  for (int32_t n0 = 0; n0 < N; n0 += 4) {
+  auto group = [&](int32_t n0) {
     float z[4 * K];
     const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
 #pragma unroll
@@ -744,6 +746,12 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
         }
       }
     }
+  };
+  for (int32_t n0 = 0; n0 < N; n0 += 4 * U) {
+    group(n0);
+#pragma unroll
+    for (int u = 1; u < U; ++u)
+      if (n0 + 4 * u < N) group(n0 + 4 * u);
   }
 }
 

@@ -41,7 +41,8 @@ def test_batch_equals_plain_bitwise():
     plain = np.array([_plain(j) for j in jobs])
     r = BatchRunner(max_jobs=64)
     for tune in (BatchTune(4, 2, 1), BatchTune(1, 4, 4), BatchTune(8, 1, 2),
-                 BatchTune(4, 7, 1, concurrent_groups=0), BatchTune(2, 14, 2)):
+                 BatchTune(4, 7, 1, concurrent_groups=0), BatchTune(2, 14, 2),
+                 BatchTune(4, 8, 1, step_unroll=2), BatchTune(8, 4, 2, 0, 2)):
         got = r.run(jobs, tune=tune)
         assert np.array_equal(got, plain), tune
 
