@@ -64,8 +64,16 @@ enum sdemc_noise {
   SDEMC_NOISE_INDEP_COMMON = 1, /* 'indep' in the jump solver: dB_i = one scalar
                                    N(0,dt) shared by all components
                                    (small-jump Gaussian, solvers.py:198-201)          */
-  SDEMC_NOISE_INDEP_FULL = 2    /* 'indep' in the diffusion solver: B is a second
+  SDEMC_NOISE_INDEP_FULL = 2,   /* 'indep' in the diffusion solver: B is a second
                                    correlated BM independent of W (solvers.py:80-81)   */
+  SDEMC_NOISE_HESTON = 3        /* Heston (sde.py:258-279, schemes.py:16-21): dim 2,
+                                   x = (S, v), correlated 'diag' increments dW;
+                                   S += a1[0] S dt + sqrt(v) S dW_0 and sqrt(v_new)
+                                   is the positive root of the drift-implicit
+                                   equation (1 + kappa dt) y^2 - xi dW_1 y
+                                   - (v + (kappa theta - xi^2/2) dt) = 0
+                                   (helpers.py:36-48); jp = {kappa, theta, xi};
+                                   no jumps, uniform grid, 2 kappa theta > xi^2  */
 };
 
 enum sdemc_jump_law {

@@ -146,7 +146,9 @@ static void model_init(model_t* m, const sdemc_job* job) {
   m->job = job;
   m->D = job->model.dim;
   m->noise = job->model.noise;
-  m->K = m->noise == SDEMC_NOISE_DIAG ? m->D : (m->noise == SDEMC_NOISE_INDEP_COMMON ? m->D + 1 : 2 * m->D);
+  m->K = (m->noise == SDEMC_NOISE_DIAG || m->noise == SDEMC_NOISE_HESTON)
+             ? m->D
+             : (m->noise == SDEMC_NOISE_INDEP_COMMON ? m->D + 1 : 2 * m->D);
   for (int i = 0; i < D_MAX; ++i)
     for (int j = 0; j < D_MAX; ++j) m->L[i][j] = job->model.chol[i * D_MAX + j];
   m->lv_y1 = (float)job->model.jp[6];
@@ -198,6 +200,21 @@ static double mark_from_words(const model_t* m, uint32_t m1, uint32_t m2) {
 /* Euler step of schemes.py:5-10 in fp64; dW, dB already scaled/correlated */
 static void euler(const model_t* m, double* x, double dt, const double* dW, const double* dB) {
   const sdemc_model* md = &m->job->model;
+  if (m->noise == SDEMC_NOISE_HESTON) {
+    /* HestonScheme.step (schemes.py:16-21): S by Euler with sqrt of the
+     * pre-step v; the new v is the square of the larger root of
+     * a y^2 + b y + c = 0, a = -1 - kappa dt, b = xi dW_1,
+     * c = v + kappa theta dt - xi^2 dt / 2 (sde.py:274-278, helpers.py:36-48) */
+    double S = x[0], v = x[1];
+    double kappa = md->jp[0], theta = md->jp[1], xi = md->jp[2];
+    double a = -1.0 - kappa * dt, b = xi * dW[1], c = v + kappa * theta * dt - 0.5 * dt * xi * xi;
+    double sd = sqrt(b * b - 4.0 * a * c);
+    double r1 = (-b + sd) / (2.0 * a), r2 = (-b - sd) / (2.0 * a);
+    double y = r1 > r2 ? r1 : r2;
+    x[0] = S + md->a1[0] * S * dt + sqrt(v) * S * dW[0];
+    x[1] = y * y;
+    return;
+  }
   for (int i = 0; i < m->D; ++i) {
     double a = md->a0[i] + md->a1[i] * x[i];
     double b = md->b0[i] + md->b1[i] * x[i];

@@ -9,17 +9,18 @@ reached through the C ABI of ``lib/libsdemc.so`` (include/sdemc.h).
 from .version import __version__
 from .helpers import (partition, solve_quadratic, mc_estimates, remove_steps, get_corr_matrix,
                       ceil_mult, get_jump_comp, sample_cov)
-from .sde import (Sde, DiffusionSde, Gbm, LogGbm, DoubleGbm, LogNormalJumpsSde, Merton,
+from .sde import (Sde, DiffusionSde, Gbm, LogGbm, DoubleGbm, Heston, LogNormalJumpsSde, Merton,
                   KernelModel)
 from .levy import UNIFORM_TOL, InverseCdf, Levy, LevySde, ExampleLevy, ExpExampleLevy, Levy2d
-from .schemes import EulerScheme
+from .schemes import EulerScheme, HestonScheme
 from .options import (bs_binary_aon, bs_call, merton_call, bs_digital_call, bs_asian_call, Option,
                       EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, AsianCall,
                       HestonRainbow, BestOf, ConstantShortRate)
 from .solvers import (SdeSolver, DiffusionSolver, JumpDiffusionSolver, EulerSolver,
-                      JumpEulerSolver)
+                      HestonSolver, JumpEulerSolver)
 from .mc import MCStatistics, mc_simple, find_num_trials, run_mc
 from .mlmc import mc_multilevel, get_optimal_trials, mlmc_bs_from_trials
-from .problem import (Problem, BlackScholesEuroCall, BlackScholesRainbow, MertonEuroCall,
+from .problem import (Problem, BlackScholesEuroCall, BlackScholesRainbow, HestonEuroCall,
+                      MertonEuroCall,
                       LevyRainbow, LevyRainbowMLMC, LevyCall, LevyBestOf, LevyCallOnMax)
 from ._lib import SdemcError, HipUnavailableError

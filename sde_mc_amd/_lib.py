@@ -14,7 +14,7 @@ CHUNK = 64
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_WORKSPACE, ERR_OVERFLOW, ERR_DEVICE = range(7)
 
-NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL = 0, 1, 2
+NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON = 0, 1, 2, 3
 JUMP_LOGNORMAL, JUMP_LEVY = 0, 1
 GRID_UNIFORM, GRID_ADAPTED = 0, 1
 (PAYOFF_EURO_CALL, PAYOFF_EURO_PUT, PAYOFF_BINARY_AON, PAYOFF_DIGITAL, PAYOFF_BASKET_ARITH,

@@ -35,9 +35,11 @@ def chunks_of(n_paths):
 
 
 def noise_width(model):
+    """Normals drawn per step."""
     d = model.dim
-    return d if model.noise == L.NOISE_DIAG else (d + 1 if model.noise == L.NOISE_INDEP_COMMON
-                                                  else 2 * d)
+    if model.noise in (L.NOISE_DIAG, L.NOISE_HESTON):
+        return d
+    return d + 1 if model.noise == L.NOISE_INDEP_COMMON else 2 * d
 
 
 # Issue cost (VALU slots per wavefront step) of one normal and of a

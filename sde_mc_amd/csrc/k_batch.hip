@@ -116,6 +116,7 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
       SDEMC_CASES_DG(2, 2, FAM_DOUBLE, 0)
       SDEMC_CASES_DG(2, 3, FAM_DOUBLE, 0)
       SDEMC_CASES_DG(2, 4, FAM_DOUBLE, 0)
+      SDEMC_CASES_DG(3, 2, FAM_HESTON, 0)
       default: return false;
     }
   }

@@ -24,7 +24,7 @@ int validate_job(const sdemc_job* job) {
   if (!job) return SDEMC_ERR_ARG;
   const sdemc_model& m = job->model;
   if (m.dim < 1 || m.dim > SDEMC_MAX_DIM) return SDEMC_ERR_UNSUPPORTED;
-  if (m.noise < SDEMC_NOISE_DIAG || m.noise > SDEMC_NOISE_INDEP_FULL) return SDEMC_ERR_UNSUPPORTED;
+  if (m.noise < SDEMC_NOISE_DIAG || m.noise > SDEMC_NOISE_HESTON) return SDEMC_ERR_UNSUPPORTED;
   if (m.jump_law != SDEMC_JUMP_LOGNORMAL && m.jump_law != SDEMC_JUMP_LEVY)
     return SDEMC_ERR_UNSUPPORTED;
   if (m.exact_jumps != 0 && m.exact_jumps != 1) return SDEMC_ERR_ARG;
@@ -69,6 +69,18 @@ int validate_job(const sdemc_job* job) {
   if (m.rate * job->T > 1.0e5) return SDEMC_ERR_ARG;
   if (m.noise == SDEMC_NOISE_INDEP_FULL && job->grid == SDEMC_GRID_ADAPTED)
     return SDEMC_ERR_UNSUPPORTED;
+  if (m.noise == SDEMC_NOISE_HESTON) {
+    // (S, v) with no jumps on the uniform grid; the Feller condition keeps the
+    // implicit variance equation's constant term positive (sde.py:260)
+    if (m.dim != 2 || job->grid != SDEMC_GRID_UNIFORM) return SDEMC_ERR_UNSUPPORTED;
+    if (m.rate != 0.0) return SDEMC_ERR_UNSUPPORTED;
+    const double kappa = m.jp[0], theta = m.jp[1], xi = m.jp[2];
+    if (!(kappa > 0.0) || !(theta > 0.0) || !(xi > 0.0) || !finite(kappa) || !finite(theta) ||
+        !finite(xi))
+      return SDEMC_ERR_ARG;
+    if (!(2.0 * kappa * theta > xi * xi)) return SDEMC_ERR_ARG;
+    if (!(m.x0[1] >= 0.0)) return SDEMC_ERR_ARG;
+  }
   return SDEMC_OK;
 }
 
@@ -163,6 +175,7 @@ static int dispatch_rng(const sdemc_job& job, const LaunchArgs& a) {
     case SDEMC_NOISE_DIAG: return launch_rng_merton(job, a);
     case SDEMC_NOISE_INDEP_COMMON: return launch_rng_levy(job, a);
     case SDEMC_NOISE_INDEP_FULL: return launch_rng_double(job, a);
+    case SDEMC_NOISE_HESTON: return launch_rng_heston(job, a);
     default: return SDEMC_ERR_UNSUPPORTED;
   }
 }
@@ -172,6 +185,7 @@ static int dispatch_fixed(const sdemc_job& job, const sdemc_fixed_in& in, const 
     case SDEMC_NOISE_DIAG: return launch_fixed_merton(job, in, a);
     case SDEMC_NOISE_INDEP_COMMON: return launch_fixed_levy(job, in, a);
     case SDEMC_NOISE_INDEP_FULL: return launch_fixed_double(job, in, a);
+    case SDEMC_NOISE_HESTON: return launch_fixed_heston(job, in, a);
     default: return SDEMC_ERR_UNSUPPORTED;
   }
 }

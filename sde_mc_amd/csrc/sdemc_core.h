@@ -30,7 +30,13 @@
 
 namespace sdemc {
 
-enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2 };
+enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2, FAM_HESTON = 3 };
+
+// Families whose increments are D correlated normals and no second driver.
+template <int FAM>
+struct DiagNoise {
+  static constexpr bool value = FAM == FAM_MERTON || FAM == FAM_HESTON;
+};
 enum { STREAM_BM = 0, STREAM_JUMP = 1, STREAM_POIS = 2, STREAM_MARK = 3 };
 
 constexpr int WAVE = 64;
@@ -167,6 +173,7 @@ struct KModel {
   int32_t exact, law, dim, noise;
   double rate;
   float inv_rate;  // 1 / lambda (fp32 jump clock)
+  float hv_kappa, hv_drift, hv_xi;  // Heston variance: kappa, kappa theta - xi^2/2, xi
 };
 
 struct KPayoff {
@@ -268,6 +275,12 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   }
   f.rate = m.rate;
   f.inv_rate = m.rate > 0.0 ? (float)(1.0 / m.rate) : 0.0f;
+  f.hv_kappa = f.hv_drift = f.hv_xi = 0.0f;
+  if (m.noise == SDEMC_NOISE_HESTON) {
+    f.hv_kappa = (float)m.jp[0];
+    f.hv_drift = (float)(m.jp[0] * m.jp[1] - 0.5 * m.jp[2] * m.jp[2]);
+    f.hv_xi = (float)m.jp[2];
+  }
   k.pay.kind = j.payoff.kind;
   k.pay.log = j.payoff.log;
   k.pay.strike = (float)j.payoff.strike;
@@ -399,6 +412,23 @@ __device__ __forceinline__ float payoff_raw(const KPayoff& p, const float (&x)[D
 template <int D, int FAM>
 __device__ __forceinline__ void euler(const KModel& f, float (&x)[D], float dt,
                                       const float (&dW)[D], const float (&dB)[D]) {
+  if constexpr (FAM == FAM_HESTON) {
+    // x = (S, v): S by Euler with the pre-step sqrt(v); sqrt of the new v is
+    // the positive root of (1 + kappa dt) y^2 - xi dW_1 y - (v + d dt) = 0,
+    // y = (B + sqrt(B^2 + 4 A C)) / (2 A) (schemes.py:16-21, helpers.py:36-48)
+    static_assert(D == 2, "Heston state is (S, v)");
+    const float S = x[0], v = x[1];
+    const float a = f.a1[0] * S;
+    const float b = __fsqrt_rn(v) * S;
+    x[0] = __fmaf_rn(b, dW[0], __fmaf_rn(a, dt, S));
+    const float A = __fmaf_rn(f.hv_kappa, dt, 1.0f);
+    const float C = __fmaf_rn(f.hv_drift, dt, v);
+    const float B = f.hv_xi * dW[1];
+    const float disc = __fmaf_rn(B, B, 4.0f * A * C);
+    const float y = __fdiv_rn(B + __fsqrt_rn(disc), 2.0f * A);
+    x[1] = y * y;
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     const float a = __fmaf_rn(f.a1[i], x[i], f.a0[i]);
@@ -424,7 +454,7 @@ __device__ __forceinline__ void add_jump(const KModel& f, float (&x)[D], const f
 // Number of normals drawn per step.
 template <int D, int FAM>
 struct NK {
-  static constexpr int K = FAM == FAM_MERTON ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+  static constexpr int K = DiagNoise<FAM>::value ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
 };
 
 // Correlated, scaled increments from K standard normals:
@@ -731,8 +761,8 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
             for (int i = 0; i < D; ++i) st.jumps[(r + i) * st.P + c.p] = jsum;
           }
           if (st.normals) {
-            const int64_t rn = (int64_t)n * D * (FAM == FAM_MERTON ? 1 : 2);
-            if (FAM == FAM_MERTON) {
+            const int64_t rn = (int64_t)n * D * (DiagNoise<FAM>::value ? 1 : 2);
+            if (DiagNoise<FAM>::value) {
 #pragma unroll
               for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + c.p] = dW[i];
             } else {
@@ -759,7 +789,7 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
 // common: the last row is the shared increment), 2D ('indep' full).
 template <int D, int FAM>
 struct NKF {
-  static constexpr int K = FAM == FAM_MERTON ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+  static constexpr int K = DiagNoise<FAM>::value ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
 };
 
 template <int D, int FAM>
@@ -1145,8 +1175,8 @@ __device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_
         }
         if (st.times) st.times[(int64_t)steps * st.P + p] = t;
         if (st.normals) {
-          const int64_t rn = (int64_t)(steps - 1) * D * (FAM == FAM_MERTON ? 1 : 2);
-          if (FAM == FAM_MERTON) {
+          const int64_t rn = (int64_t)(steps - 1) * D * (DiagNoise<FAM>::value ? 1 : 2);
+          if (DiagNoise<FAM>::value) {
 #pragma unroll
             for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + p] = dW[i];
           } else {

@@ -176,7 +176,7 @@ int launch_rng_d(const sdemc_job& sj, const LaunchArgs& a) {
 #define SDEMC_L(G, C, S)                                                                  \
   hipLaunchKernelGGL((k_rng<D, FAM, G, C, S>), g, b, 0, a.stream, job, a.partials, a.cflags, \
                      a.out, nchunks)
-  if constexpr (FAM == FAM_DOUBLE) {
+  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON) {
     if (adapted) return SDEMC_ERR_UNSUPPORTED;
     if (coupled) SDEMC_L(SDEMC_GRID_UNIFORM, true, false);
     else if (a.store) SDEMC_L(SDEMC_GRID_UNIFORM, false, true);
@@ -214,7 +214,7 @@ int launch_fixed_d(const sdemc_job& sj, const sdemc_fixed_in& in, const LaunchAr
 #define SDEMC_L(G, C)                                                                        \
   hipLaunchKernelGGL((k_fixed<D, FAM, G, C>), g, b, 0, a.stream, job, in, a.partials, a.cflags, \
                      a.out, nchunks)
-  if constexpr (FAM == FAM_DOUBLE) {
+  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON) {
     if (adapted) return SDEMC_ERR_UNSUPPORTED;
     if (coupled) SDEMC_L(SDEMC_GRID_UNIFORM, true);
     else SDEMC_L(SDEMC_GRID_UNIFORM, false);
@@ -260,5 +260,7 @@ int launch_rng_double(const sdemc_job& job, const LaunchArgs& a);
 int launch_fixed_merton(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
 int launch_fixed_levy(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
 int launch_fixed_double(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
+int launch_rng_heston(const sdemc_job& job, const LaunchArgs& a);
+int launch_fixed_heston(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
 
 }  // namespace sdemc

@@ -20,7 +20,7 @@ import torch
 
 from . import _lib as L
 from . import engine as E
-from .schemes import EulerScheme
+from .schemes import EulerScheme, HestonScheme
 
 
 class SdeSolver(ABC):
@@ -264,6 +264,10 @@ def _multilevel(solver, bs, levels):
 
 class EulerSolver(EulerScheme, DiffusionSolver):
     """Euler scheme on the fixed grid (solvers.py:122-123)."""
+
+
+class HestonSolver(HestonScheme, DiffusionSolver):
+    """Heston scheme on the fixed grid (solvers.py:126-127)."""
 
 
 class JumpEulerSolver(EulerScheme, JumpDiffusionSolver):

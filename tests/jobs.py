@@ -47,7 +47,9 @@ def job_for(sde, T, steps, n_paths, grid='uniform', seed=1234, path_offset=0, co
 
 def n_normals(job):
     d, nz = job.model.dim, job.model.noise
-    return d if nz == L.NOISE_DIAG else (d + 1 if nz == L.NOISE_INDEP_COMMON else 2 * d)
+    if nz in (L.NOISE_DIAG, L.NOISE_HESTON):
+        return d
+    return d + 1 if nz == L.NOISE_INDEP_COMMON else 2 * d
 
 
 def mark_sampler(job, rng):

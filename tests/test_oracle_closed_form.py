@@ -2,6 +2,7 @@
 (tests/test_options.py of sde_mc) and against independent computations."""
 import math
 
+import numpy as np
 import pytest
 
 from oracle import closed_form as cf
@@ -79,3 +80,37 @@ def test_oracle_payoffs_match_reference_option_kats():
         pay, term, _ = oracle.simulate(frozen_state_job(kind, K, x), 0, 1)
         assert np.allclose(term[0, :, 0], np.float32(x)), (kind, x)
         assert abs(pay[0] - want) < 1e-6, (kind, x, pay[0], want)
+
+
+HESTON = (1.0, 1.0, 1.0, 0.02, 0.04, 1.5, 0.04, 0.3, -0.7)   # S K T r v0 kappa theta xi rho
+
+
+def test_lewis_integral_matches_black_scholes():
+    for S, K, T, r, s in ((1.0, 0.9, 1.0, 0.02, 0.25), (1.0, 1.3, 2.0, 0.05, 0.4)):
+        bscf = lambda u: np.exp(1j * u * (math.log(S) + (r - 0.5 * s * s) * T) - 0.5 * u * u * s * s * T)
+        assert abs(cf.lewis_call(S, K, T, r, bscf) - cf.bs_call(S, K, T, r, s)) < 1e-10
+
+
+@pytest.mark.parametrize("K", [0.7, 1.0, 1.4])
+def test_heston_two_integrations_agree(K):
+    S, _, T, r, v0, ka, th, xi, rho = HESTON
+    a = cf.heston_call(S, K, T, r, v0, ka, th, xi, rho)
+    b = cf.lewis_call(S, K, T, r, lambda u: cf.heston_cf(u, S, T, r, v0, ka, th, xi, rho))
+    assert abs(a - b) < 1e-10
+
+
+def test_heston_limits_and_parity():
+    S, K, T, r, v0, ka, th, xi, rho = HESTON
+    # martingale: E[S_T] = S e^{rT}
+    assert abs(cf.heston_cf(-1j, S, T, r, v0, ka, th, xi, rho) - S * math.exp(r * T)) < 1e-12
+    # vanishing vol of vol with v0 = theta: Black-Scholes at sigma^2 = theta,
+    # approached linearly in xi
+    d3 = cf.heston_call(S, 0.9, T, r, th, ka, th, 1e-3, rho) - cf.bs_call(S, 0.9, T, r, math.sqrt(th))
+    d4 = cf.heston_call(S, 0.9, T, r, th, ka, th, 1e-4, rho) - cf.bs_call(S, 0.9, T, r, math.sqrt(th))
+    assert abs(d4) < 2e-6 and abs(d3 / d4 - 10.0) < 0.5
+    # put-call parity through the closed-form put (K e^{-rT} - S + C)
+    for K_ in (0.8, 1.2):
+        c = cf.heston_call(S, K_, T, r, v0, ka, th, xi, rho)
+        p_int = cf.lewis_call(S, K_, T, r, lambda u: cf.heston_cf(u, S, T, r, v0, ka, th, xi, rho)) \
+            - S + K_ * math.exp(-r * T)
+        assert p_int > 0 and abs(c - (p_int + S - K_ * math.exp(-r * T))) < 1e-10

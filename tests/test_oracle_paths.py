@@ -334,3 +334,41 @@ def test_per_step_counts_are_poisson_and_marks_follow_the_levy_law():
     # ... independently of the count: the first mark of multi-jump steps too
     assert len(firsts_multi) > 500
     assert stats.kstest(np.array(firsts_multi), cdf).pvalue > 1e-3
+
+
+def _heston_job(N, P, xi=0.3, rho=-0.7, seed=21, T=1.0):
+    import sde_mc_amd as sm
+    sde = sm.Heston(0.02, 1.5, 0.04, xi, rho, torch.tensor([1.0, 0.04]))
+    return J.job_for(sde, T, N, P, option=sm.EuroCall(1.0), disc=math.exp(-0.02 * T), seed=seed)
+
+
+def test_heston_zero_vol_of_vol_product_formula():
+    # xi = 0: the implicit variance equation gives v' = (v + kappa theta h) / (1 + kappa h)
+    # (implicit Euler of dv = kappa (theta - v) dt), and S_{n+1} = S_n (1 + r h + sqrt(v_n) dW_n)
+    P, N = 6, 20
+    job = _heston_job(N, P, xi=0.3)
+    job.model.jp[2] = 0.0          # the oracle takes any xi; the kernels require xi > 0
+    job.model.x0[1] = 0.09
+    fx = J.fixed_uniform(job, seed=5)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    h = 1.0 / N
+    v, S = 0.09, np.ones(P)
+    dw = fx.a['dw'].astype(np.float64)
+    for n in range(N):
+        S = S * (1 + 0.02 * h + math.sqrt(v) * dw[n, 0])
+        v = (v + 1.5 * 0.04 * h) / (1 + 1.5 * h)
+    assert np.allclose(term[0, 0], S, rtol=1e-13) and np.allclose(term[0, 1], v, rtol=1e-13)
+
+
+def test_heston_oracle_price_extrapolates_to_the_closed_form():
+    # first-order weak error: (4 p(4N) - p(N)) / 3 removes the h term; common
+    # random numbers (same seed) keep the two runs' noise correlated
+    from oracle import closed_form as cf
+    want = cf.heston_call(1.0, 1.0, 1.0, 0.02, 0.04, 1.5, 0.04, 0.3, -0.7)
+    P = 1 << 18
+    p32, _, _ = oracle.simulate(_heston_job(32, P), 0, P)
+    p128, _, _ = oracle.simulate(_heston_job(128, P), 0, P)
+    extrap = (4 * p128.mean() - p32.mean()) / 3
+    se = np.std(4 * p128 - p32) / 3 / math.sqrt(P)
+    assert abs(extrap - want) < 4 * se + 1e-4, (extrap, want, se)
+    assert p32.mean() < p128.mean() < want + 4 * se  # the bias shrinks with h
