@@ -50,6 +50,18 @@ struct U4 {
   uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c for a wave-uniform c: one gfx950 3-input bitwise op (truth
+// table 0x96 = odd parity) instead of two xors.
+__host__ __device__ __forceinline__ uint32_t xor3_key(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t d;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                       uint32_t c3, uint32_t k0, uint32_t k1) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -66,9 +78,9 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, u
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c0 = hi1 ^ c1 ^ k0;
+    c0 = xor3_key(hi1, c1, k0);
     c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
+    c2 = xor3_key(hi0, c3, k1);
     c3 = lo0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
