@@ -228,6 +228,72 @@ int sdemc_reduce_finalize(const double* partials_dev, const int32_t* chunk_flags
                           const int64_t* chunk_base_dev, const uint64_t* n_paths_dev,
                           int32_t n_jobs, double* stats_dev, void* stream);
 
+/* ---- neural control variates (mc.py:126-326, varred.py:77-131, 202-214) -
+ * ReLU MLPs evaluated with bf16 matrix-core (MFMA) products and fp32
+ * accumulation at every step of every path give the control-variate
+ * estimators of apply_diffusion_control_variate (uniform grid, no jumps) and
+ * apply_adapted_control_variates (jump-adapted grid):
+ *   gamma = disc(T) payoff(X_T)
+ *         + sum_{n < cv_steps} disc(t_n) <f(t_n, X_n), dW_n>          (Brownian)
+ *         + sum_n disc(t_n) <g(t_n, X_n-), 1> J_n                     (jumps)
+ *         - jump_comp sum_n disc(t_n) <g(t_n, X_n-), 1> (t_{n+1} - t_n)
+ * with disc(t) = exp(-rate t), X_n the state at grid time t_n (after any
+ * jump there), X_n- the state before it, J_n the jump mark applied at t_n
+ * (0 at most times) and dW_n the step's correlated increments ('diag': dim
+ * components; 'indep': dim pairs (dW_i, dB_i) matched with f outputs
+ * 2i, 2i + 1).  The control variate has mean zero for any f and g (up to the
+ * time discretisation of the compensator), so rounding the networks to bf16
+ * costs variance reduction, never bias.  Paths are those of sdemc_simulate
+ * for the same job (same Philox counters): f = g = 0 gives its payoffs
+ * exactly.  Every row of every path is included; the reference truncates a
+ * batch at its longest path's final row (varred.py:113-118 on
+ * nets.py:192-207).
+ *
+ * Jobs: single level; uniform grid without jumps ('diag' dim 1..4, 'indep'
+ * Double-GBM dim 1..4, Heston) or adapted grid with lognormal or Levy jumps
+ * ('diag' / 'indep', dim 1..4); 'indep' needs 2 dim <= 8 f outputs.
+ *
+ * A network arrives as bf16 MFMA operand words built by the host
+ * (sde_mc_amd/cv.py, pack_network): hidden layers of at most 63 units padded
+ * to 64, the first padded unit a constant 1 carrying the next layer's bias;
+ * inputs (t, x_1..x_dim, 1) padded to 8 and fed as bf16 high and low parts
+ * (the first layer's weights are shared by both); outputs padded to 32.
+ * The words form blocks of 512, word 512 b + 8 l + q of block b holding for
+ * lane l = 0..63 and q = 0..7 the transposed weight W^T[o][i] (output unit
+ * o, input unit i):
+ *   first layer, blocks b = m (m = 0, 1):        W^T[32 m + l % 32][q]
+ *   hidden layer k = 1..L-1, b = 2 + 8 (k - 1) + 4 m + c (m = 0, 1; c = 0..3):
+ *                                                W^T[32 m + l % 32][u(c, l / 32, q)]
+ *   output layer, b = 2 + 8 (L - 1) + c (c = 0..3): W^T[l % 32][u(c, l / 32, q)]
+ * with u(c, h, q) = 16 c + 8 (q / 4) + 4 h + q % 4, the order in which a
+ * 32x32 accumulator tile's rows sit in the lanes: 8 L - 2 blocks in all.
+ */
+#define SDEMC_CV_MAX_HIDDEN 4
+#define SDEMC_CV_MAX_OUT 8
+typedef struct sdemc_cv_mlp {
+  const uint16_t* weights_dev; /* sdemc_cv_mlp_words(hidden_layers) bf16 words */
+  int32_t hidden_layers;       /* 1..SDEMC_CV_MAX_HIDDEN; 0 = the zero network */
+  int32_t n_out;               /* f: brown_dim, g: dim                         */
+} sdemc_cv_mlp;
+
+typedef struct sdemc_cv_spec {
+  sdemc_cv_mlp f;    /* Brownian integrand f(t, X_t)                         */
+  sdemc_cv_mlp g;    /* jump integrand g(t, X_t-); adapted grid only          */
+  int32_t cv_steps;  /* Brownian terms of steps n < cv_steps (remove_steps)   */
+  int32_t reserved;
+  double rate;       /* discount exp(-rate t) of the integrands               */
+  double jump_comp;  /* jump_rate * jump_mean of the compensator              */
+} sdemc_cv_spec;
+
+/* bf16 words of a packed network with this many hidden layers. */
+size_t sdemc_cv_mlp_words(int32_t hidden_layers);
+
+/* Control-variate run: stats_dev gets {sum, sum of squares, n, flags} of
+ * gamma; gamma_dev (optional, [P] fp32) the per-path values. */
+int sdemc_cv_simulate(const sdemc_job* job, const sdemc_cv_spec* cv, double* stats_dev,
+                      float* gamma_dev, void* workspace_dev, size_t workspace_bytes,
+                      void* stream);
+
 /* ---- batched persistent launch ------------------------------------------
  * A job table of max_jobs slots lives on the device in the library's own
  * prepared layout (sdemc_sizeof_prepared_job() bytes per slot), filled by

@@ -18,7 +18,15 @@ from .options import (bs_binary_aon, bs_call, merton_call, bs_digital_call, bs_a
                       HestonRainbow, BestOf, ConstantShortRate)
 from .solvers import (SdeSolver, DiffusionSolver, JumpDiffusionSolver, EulerSolver,
                       HestonSolver, JumpEulerSolver)
-from .mc import MCStatistics, mc_simple, find_num_trials, run_mc
+from .mc import (MCStatistics, mc_simple, find_num_trials, run_mc, mc_control_variates,
+                 mc_apply_cvs, mc_adaptive_cv, mc_terminal_cv, simulate_data,
+                 simulate_adapted_data, sim_train_control_variates, sample_batch_cost,
+                 find_num_trials_terminal_cv, run_mc_terminal_cv, run_cv_mc)
+from .nets import (ControlVariate, ZeroFunction, Mlp, Lstm, Gru, NormalPathData,
+                   NormalJumpsPathData, AdaptedPathData, get_mlps, get_opt)
+from .varred import (EarlyStopping, train_diffusion_control_variate,
+                     apply_diffusion_control_variate, apply_adapted_control_variates,
+                     train_adapted_control_variates, integrate_cv)
 from .mlmc import mc_multilevel, get_optimal_trials, mlmc_bs_from_trials
 from .problem import (Problem, BlackScholesEuroCall, BlackScholesRainbow, HestonEuroCall,
                       MertonEuroCall,

@@ -127,6 +127,10 @@ _SIGS = {
                                              ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                              ctypes.c_void_p]),
     "sdemc_job_class": (ctypes.c_int, [ctypes.POINTER(Job)]),
+    "sdemc_cv_mlp_words": (ctypes.c_size_t, [ctypes.c_int32]),
+    "sdemc_cv_simulate": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                         ctypes.c_void_p]),
     "sdemc_prepare_jobs": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int32)]),

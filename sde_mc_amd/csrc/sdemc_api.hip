@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include "sdemc_kernels.h"
+#include "cv_mlp.h"
 
 namespace sdemc {
 
@@ -219,6 +220,64 @@ int sdemc_simulate(const sdemc_job* job, double* stats_dev, const sdemc_path_out
   if (a.store && job->coarse_factor > 0) return SDEMC_ERR_UNSUPPORTED;
   if (nc > 0) {
     st = dispatch_rng(*job, a);
+    if (st != SDEMC_OK) return st;
+  }
+  return finalize_plain(*job, a.partials, a.cflags, stats_dev, a.stream);
+}
+
+size_t sdemc_cv_mlp_words(int32_t hidden_layers) {
+  if (hidden_layers < 1 || hidden_layers > SDEMC_CV_MAX_HIDDEN) return 0;
+  return (size_t)cv_words(hidden_layers);
+}
+
+static int check_mlp(const sdemc_cv_mlp& n, int n_out) {
+  if (n.hidden_layers == 0) return SDEMC_OK;  // the zero network
+  if (n.hidden_layers < 0 || n.hidden_layers > SDEMC_CV_MAX_HIDDEN) return SDEMC_ERR_ARG;
+  if (!n.weights_dev || ((uintptr_t)n.weights_dev & 15u) != 0) return SDEMC_ERR_ARG;
+  if (n.n_out != n_out) return SDEMC_ERR_ARG;
+  return SDEMC_OK;
+}
+
+int sdemc_cv_simulate(const sdemc_job* job, const sdemc_cv_spec* cv, double* stats_dev,
+                      float* gamma_dev, void* workspace_dev, size_t workspace_bytes_,
+                      void* stream) {
+  int st = validate_job(job);
+  if (st != SDEMC_OK) return st;
+  if (!cv || !stats_dev || !workspace_dev) return SDEMC_ERR_ARG;
+  const sdemc_model& m = job->model;
+  if (job->coarse_factor != 0) return SDEMC_ERR_UNSUPPORTED;
+  const bool adapted = job->grid == SDEMC_GRID_ADAPTED;
+  if (adapted) {
+    if (m.noise != SDEMC_NOISE_DIAG && m.noise != SDEMC_NOISE_INDEP_COMMON)
+      return SDEMC_ERR_UNSUPPORTED;
+  } else {
+    if (m.rate != 0.0 || m.noise == SDEMC_NOISE_INDEP_COMMON) return SDEMC_ERR_UNSUPPORTED;
+  }
+  const bool diag = m.noise == SDEMC_NOISE_DIAG || m.noise == SDEMC_NOISE_HESTON;
+  const int brown = diag ? m.dim : 2 * m.dim;
+  if (brown > SDEMC_CV_MAX_OUT || m.dim + 2 > 8) return SDEMC_ERR_UNSUPPORTED;
+  if ((st = check_mlp(cv->f, brown)) != SDEMC_OK) return st;
+  if ((st = check_mlp(cv->g, m.dim)) != SDEMC_OK) return st;
+  if (cv->g.hidden_layers != 0 && !adapted) return SDEMC_ERR_ARG;
+  const int32_t limit = adapted ? job->max_steps : job->steps;
+  if (cv->cv_steps < 0 || cv->cv_steps > limit) return SDEMC_ERR_ARG;
+  if (!isfinite(cv->rate) || !isfinite(cv->jump_comp)) return SDEMC_ERR_ARG;
+  if (workspace_bytes_ < workspace_bytes(job->n_paths)) return SDEMC_ERR_WORKSPACE;
+  const int64_t nc = n_chunks(job->n_paths);
+  CvLaunch a;
+  a.fw = cv->f.weights_dev;
+  a.f_hidden = cv->f.hidden_layers;
+  a.gw = cv->g.weights_dev;
+  a.g_hidden = cv->g.hidden_layers;
+  a.rate = (float)cv->rate;
+  a.comp = (float)cv->jump_comp;
+  a.cv_steps = cv->cv_steps;
+  a.partials = (double*)workspace_dev;
+  a.cflags = (int32_t*)((char*)workspace_dev + (size_t)nc * 2 * sizeof(double));
+  a.gamma = gamma_dev;
+  a.stream = (hipStream_t)stream;
+  if (nc > 0) {
+    st = adapted ? launch_cv_adapted(*job, a) : launch_cv_uniform(*job, a);
     if (st != SDEMC_OK) return st;
   }
   return finalize_plain(*job, a.partials, a.cflags, stats_dev, a.stream);
