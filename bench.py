@@ -229,6 +229,98 @@ def mlmc_leg(tune, dev, reps=5):
             "error": est - MERTON_CLOSED_FORM}
 
 
+def _mlp_flops(layers):
+    """Useful flops of one evaluation of an MLP with these widths (biases included)."""
+    return sum(2 * a * b + b for a, b in zip(layers[:-1], layers[1:]))
+
+
+def cv_leg(dev, reps=3, P=1 << 20, N=1000, P_torch=32000):
+    """Neural control variates on the reference's Merton experiment
+    (examples/merton_1d_european/merton_cv_experiment.py): Merton 1-D call,
+    T=3, N steps on the jump-adapted grid, f and g = MLP 2-50-50-50-1
+    (random weights; the cost does not depend on them), P paths per fused
+    launch.  Compared with the reference's route on the same device: paths
+    stored by the HIP kernel, then apply_adapted_control_variates in torch
+    (fp32 GEMMs) in network batches of 2000, P_torch paths."""
+    import torch
+    import sde_mc_amd as sm
+    from sde_mc_amd import cv as CV
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import mc as MC
+    from sde_mc_amd import _lib as L
+    p = sm.MertonEuroCall.default_params(N, 'cuda')
+    torch.manual_seed(0)
+    nets = [sm.Mlp(2, [50, 50, 50], 1, batch_norm=False, batch_norm_init=False,
+                   device='cuda').eval() for _ in range(2)]
+    fused = MC.fused_control_variates(nets, p.solver, p.payoff, p.discounter)
+    disc = float(p.discounter(p.solver.time_interval))
+    job, _ = p.solver.make_job(P, option=p.payoff, discount=disc)
+    ns = torch.empty(P, dtype=torch.int32, device=dev)
+    out = L.PathOut()
+    out.nsteps_dev = ns.data_ptr()
+    plain_stats = E.simulate(job, dev, out=out)
+    plain_ms = _time_events(lambda: E.simulate(job, dev, stats=plain_stats), reps)
+    path_steps = int(ns.to(torch.int64).sum())
+    steps_max = job.max_steps
+
+    def run():
+        return CV.cv_simulate(job, fused.f, fused.g, steps_max, fused.rate, fused.comp, dev)[0]
+    st = run()
+    torch.cuda.synchronize()
+    ms = _time_events(run, reps)
+    s, q, n = E.stats_to_host(st, 'cv bench')
+    mean, se = E.mean_and_se(s, q, n)
+    flops = path_steps * 2 * _mlp_flops([2, 50, 50, 50, 1])
+    mfma_instr = 16 * 3 - 4                      # 32x32x16 products per wave per evaluation
+    waves = (P + 63) // 64
+    # executed products: every row of the wave's longest path, both networks
+    # (rows ~ path steps here: the steps per path vary by a few jumps)
+    exec_flops = 2 * mfma_instr * 32 * 32 * 16 * 2 * (path_steps / P) * waves
+    # reference route: store, then torch
+    p2 = sm.MertonEuroCall.default_params(N, 'cuda')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dl = MC.simulate_adapted_data(P_torch, p2.solver, p2.payoff, p2.discounter, bs=2000,
+                                  inference=True)
+    x, y = sm.apply_adapted_control_variates(nets, dl, p2.solver, p2.discounter)
+    float(x)
+    torch.cuda.synchronize()
+    dt_torch = time.perf_counter() - t0
+    torch_ps = P_torch * (path_steps / P) / dt_torch
+    fused_ps = path_steps / (ms * 1e-3)
+    # diffusion form: Black-Scholes call, f only, uniform grid
+    b = sm.BlackScholesEuroCall.default_params(N, 'cuda')
+    fb = MC.fused_control_variates(nets[0], b.solver, b.payoff, b.discounter)
+    jb, _ = b.solver.make_job(P, option=b.payoff, discount=float(b.discounter(3.0)))
+    bplain = E.simulate(jb, dev)
+    bplain_ms = _time_events(lambda: E.simulate(jb, dev, stats=bplain), reps)
+
+    def runb():
+        return CV.cv_simulate(jb, fb.f, None, N, fb.rate, 0.0, dev)[0]
+    runb()
+    bms = _time_events(runb, reps)
+    bflops = P * N * _mlp_flops([2, 50, 50, 50, 1])
+    diffusion = {"workload": "Black-Scholes call T=3, %d steps, uniform grid, f MLP 2-50-50-50-1, "
+                             "%d paths" % (N, P),
+                 "path_steps_s": P * N / (bms * 1e-3), "kernel_ms": bms,
+                 "plain_kernel_ms": bplain_ms, "cv_overhead_vs_plain": bms / bplain_ms,
+                 "useful_tflops": bflops / (bms * 1e-3) / 1e12,
+                 "executed_tflops": mfma_instr * 32 * 32 * 16 * 2 * N * waves / (bms * 1e-3) / 1e12,
+                 "kernel": "k_cv_uniform<1,merton>"}
+    return {"workload": "Merton 1-D call T=3, %d steps, jump-adapted grid, f and g MLP "
+                        "2-50-50-50-1 (bf16 MFMA, fp32 accumulation), %d paths" % (N, P),
+            "path_steps_s": fused_ps, "kernel_ms": ms, "path_steps": path_steps,
+            "plain_kernel_ms": plain_ms, "cv_overhead_vs_plain": ms / plain_ms,
+            "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
+                         "peak": 2500.0, "unit": "TFLOP/s (bf16, useful flops)",
+                         "frac": flops / (ms * 1e-3) / 2.5e15,
+                         "executed_tflops": exec_flops / (ms * 1e-3) / 1e12,
+                         "kernel": "k_cv_adapted<1,merton>"},
+            "torch_route_path_steps_s": torch_ps,
+            "speedup_vs_torch_route": fused_ps / torch_ps,
+            "price": mean, "se": se, "diffusion": diffusion}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +338,8 @@ def main():
                          "concurrent groups, step unroll) instead of autotuning")
     ap.add_argument("--fixed-only", action="store_true",
                     help="run only the fixed-increment leg and print its record")
+    ap.add_argument("--cv-only", action="store_true",
+                    help="run only the control-variate leg and print its record")
     ap.add_argument("--probe-copy-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -270,6 +364,9 @@ def main():
         del src, dst
     if args.fixed_only:
         print(json.dumps({"fixed_increment": fixed_increment_leg(dev, args.steps)}))
+        return
+    if args.cv_only:
+        print(json.dumps({"control_variates": cv_leg(dev, args.steps)}))
         return
     reqs = W.request_mix(args.requests, seed=args.seed)
     jobs = [W.request_job(r) for r in reqs]
@@ -344,6 +441,7 @@ def main():
         extra["fixed_increment"] = fixed_increment_leg(dev, args.steps)
         extra["equal_ci"] = equal_ci_leg(reqs, tune, dev)
         extra["mlmc"] = mlmc_leg(tune, dev)
+        extra["control_variates"] = cv_leg(dev)
 
     cpu = None if args.no_cpu else cpu_baseline(reqs)
 

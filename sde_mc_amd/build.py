@@ -29,13 +29,21 @@ DEVICE_FLAGS = [
 ]
 
 
+# Per-file additions.  The control-variate kernels keep MFMA accumulators in
+# ordinary VGPRs (no accumulator-register round trips before the ReLU).
+FILE_FLAGS = {
+    "k_cv_uniform.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "k_cv_adapted.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def lib_path():
     return os.path.join(LIBDIR, LIBNAME)
 
 
 def _compile(src):
     obj = os.path.join(BUILDDIR, os.path.splitext(os.path.basename(src))[0] + ".o")
-    cmd = [HIPCC] + DEVICE_FLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + DEVICE_FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s" % (src, r.stderr))

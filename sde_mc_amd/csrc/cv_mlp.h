@@ -53,15 +53,20 @@ __device__ __forceinline__ uint32_t pack2(float a, float b) {
   return __builtin_bit_cast(uint32_t, v);
 }
 
-// ReLU (median of v, 0, +inf) of accumulators 8s..8s+7 as a bf16 operand.
+// ReLU of accumulators 8s..8s+7 as a bf16 operand: round to bf16 pairs, then
+// a packed signed 16-bit max with 0 (a negative bf16 is a negative int16,
+// and rounding keeps the sign, so this is relu before rounding, bit for bit).
+__device__ __forceinline__ uint32_t relu_pair(uint32_t v) {
+  uint32_t r;
+  asm("v_pk_max_i16 %0, %1, 0" : "=v"(r) : "v"(v));
+  return r;
+}
+
 template <int S>
 __device__ __forceinline__ bf16x8_t relu_chunk(const f32x16_t& c) {
-  const float inf = __builtin_inff();
   u32x4_t u;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-    u[i] = pack2(__builtin_amdgcn_fmed3f(c[8 * S + 2 * i], 0.0f, inf),
-                 __builtin_amdgcn_fmed3f(c[8 * S + 2 * i + 1], 0.0f, inf));
+  for (int i = 0; i < 4; ++i) u[i] = relu_pair(pack2(c[8 * S + 2 * i], c[8 * S + 2 * i + 1]));
   return __builtin_bit_cast(bf16x8_t, u);
 }
 
