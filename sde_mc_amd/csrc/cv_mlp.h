@@ -95,6 +95,27 @@ __device__ __forceinline__ void cv_inputs(const float (&F)[8], int lane, bf16x8_
   in1 = __builtin_bit_cast(bf16x8_t, low ? recv : lo_w);
 }
 
+// One further hidden layer (blocks blk .. blk + 7) from activations a to z,
+// both path tiles.
+__device__ __forceinline__ void cv_layer(int blk, int lane, const bf16x8_t (&a0)[4],
+                                         const bf16x8_t (&a1)[4], bf16x8_t (&z0)[4],
+                                         bf16x8_t (&z1)[4]) {
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    f32x16_t c0 = {}, c1 = {};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const bf16x8_t w = cv_frag(blk, 4 * m + c, lane);
+      c0 = mfma16(w, a0[c], c0);
+      c1 = mfma16(w, a1[c], c1);
+    }
+    z0[2 * m] = relu_chunk<0>(c0);
+    z0[2 * m + 1] = relu_chunk<1>(c0);
+    z1[2 * m] = relu_chunk<0>(c1);
+    z1[2 * m + 1] = relu_chunk<1>(c1);
+  }
+}
+
 // Network with `hidden` hidden layers whose blocks start at block `base`:
 // out[q], q < 8, is output unit q for this lane's path.
 __device__ __forceinline__ void cv_mlp(int base, int hidden, int lane, const float (&F)[8],
@@ -115,23 +136,19 @@ __device__ __forceinline__ void cv_mlp(int base, int hidden, int lane, const flo
       b1[2 * m + 1] = relu_chunk<1>(c1);
     }
   }
+  // further hidden layers in pairs, the activations alternating between b
+  // and n so no copy is needed; an odd one out copies back once
   int blk = base + 2;
-  for (int k = 1; k < hidden; ++k) {
+  int k = 1;
+  for (; k + 1 < hidden; k += 2) {
     bf16x8_t n0[4], n1[4];
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      f32x16_t c0 = {}, c1 = {};
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const bf16x8_t a = cv_frag(blk, 4 * m + c, lane);
-        c0 = mfma16(a, b0[c], c0);
-        c1 = mfma16(a, b1[c], c1);
-      }
-      n0[2 * m] = relu_chunk<0>(c0);
-      n0[2 * m + 1] = relu_chunk<1>(c0);
-      n1[2 * m] = relu_chunk<0>(c1);
-      n1[2 * m + 1] = relu_chunk<1>(c1);
-    }
+    cv_layer(blk, lane, b0, b1, n0, n1);
+    cv_layer(blk + 8, lane, n0, n1, b0, b1);
+    blk += 16;
+  }
+  if (k < hidden) {
+    bf16x8_t n0[4], n1[4];
+    cv_layer(blk, lane, b0, b1, n0, n1);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       b0[c] = n0[c];
