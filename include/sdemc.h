@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDEMC_ABI_VERSION 1
+#define SDEMC_ABI_VERSION 2
 #define SDEMC_MAX_DIM 4
 #define SDEMC_CHUNK 64 /* paths per reduction chunk (one wavefront) */
 
@@ -96,6 +96,9 @@ typedef struct sdemc_model {
   double j0[SDEMC_MAX_DIM], j1[SDEMC_MAX_DIM];
   double chol[SDEMC_MAX_DIM * SDEMC_MAX_DIM]; /* lower Cholesky factor, row-major */
   double jp[12];                              /* jump-law parameters            */
+  double ax[SDEMC_MAX_DIM]; /* drift of component i gains ax[i] * x_0 ('diag' noise,
+                               ax[0] = 0): AsianWrapper's running integral
+                               dA = S dt (sde.py:378-406)                      */
 } sdemc_model;
 
 /* ---- payoff (options.py:156-321) ---------------------------------------- */
@@ -108,7 +111,10 @@ enum sdemc_payoff_kind {
   SDEMC_PAYOFF_BASKET_GEO = 5,   /* options.py:247-259 */
   SDEMC_PAYOFF_RAINBOW = 6,    /* call on max, options.py:262-272 */
   SDEMC_PAYOFF_BEST_OF = 7,    /* max(max_i S_i, K), options.py:314-321 */
-  SDEMC_PAYOFF_IDENTITY = 8    /* x_0 itself (martingale checks)   */
+  SDEMC_PAYOFF_IDENTITY = 8,   /* x_0 itself (martingale checks)   */
+  SDEMC_PAYOFF_ASIAN_CALL = 9, /* call on s_1 / param, param = the averaging time
+                                  (exponentiated when log), options.py:287-298 */
+  SDEMC_PAYOFF_RAINBOW_EVEN = 10 /* call on max of s_0, s_2, ..., options.py:301-311 */
 };
 
 typedef struct sdemc_payoff {
@@ -117,6 +123,7 @@ typedef struct sdemc_payoff {
   double strike;
   double scale;    /* Option.discount: spot multiplier (options.py:169)         */
   double disc;     /* discounter(T) applied to the payoff (mc.py:93)            */
+  double param;    /* kind-specific: ASIAN_CALL's averaging time                */
 } sdemc_payoff;
 
 /* ---- one pricing job ------------------------------------------------------ */

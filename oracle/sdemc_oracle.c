@@ -215,8 +215,9 @@ static void euler(const model_t* m, double* x, double dt, const double* dW, cons
     x[1] = y * y;
     return;
   }
+  const double s0 = x[0]; /* pre-step x_0: AsianWrapper's integrand (sde.py:389-390) */
   for (int i = 0; i < m->D; ++i) {
-    double a = md->a0[i] + md->a1[i] * x[i];
+    double a = md->a0[i] + md->a1[i] * x[i] + (i > 0 ? md->ax[i] * s0 : 0.0);
     double b = md->b0[i] + md->b1[i] * x[i];
     double diff = b * dW[i];
     if (m->noise != SDEMC_NOISE_DIAG) diff += (md->c0[i] + md->c1[i] * x[i]) * dB[i];
@@ -279,6 +280,14 @@ static double payoff_raw(const sdemc_payoff* p, int D, const double* x) {
       mx = s[0];
       for (int i = 1; i < D; ++i) mx = s[i] > mx ? s[i] : mx;
       return mx > K ? mx : K;
+    case SDEMC_PAYOFF_ASIAN_CALL: /* options.py:295-298 on the transformed state */
+      v = s[D > 1 ? 1 : 0] / p->param;
+      if (p->log) v = exp(v);
+      return v > K ? v - K : 0.0;
+    case SDEMC_PAYOFF_RAINBOW_EVEN: /* options.py:305-311 */
+      mx = s[0];
+      for (int i = 2; i < D; i += 2) mx = s[i] > mx ? s[i] : mx;
+      return mx > K ? mx - K : 0.0;
     default: return s[0];
   }
 }

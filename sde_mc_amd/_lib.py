@@ -14,11 +14,14 @@ CHUNK = 64
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_WORKSPACE, ERR_OVERFLOW, ERR_DEVICE = range(7)
 
+ABI_VERSION = 2   # SDEMC_ABI_VERSION these structures follow
+
 NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON = 0, 1, 2, 3
 JUMP_LOGNORMAL, JUMP_LEVY = 0, 1
 GRID_UNIFORM, GRID_ADAPTED = 0, 1
 (PAYOFF_EURO_CALL, PAYOFF_EURO_PUT, PAYOFF_BINARY_AON, PAYOFF_DIGITAL, PAYOFF_BASKET_ARITH,
- PAYOFF_BASKET_GEO, PAYOFF_RAINBOW, PAYOFF_BEST_OF, PAYOFF_IDENTITY) = range(9)
+ PAYOFF_BASKET_GEO, PAYOFF_RAINBOW, PAYOFF_BEST_OF, PAYOFF_IDENTITY, PAYOFF_ASIAN_CALL,
+ PAYOFF_RAINBOW_EVEN) = range(11)
 
 _D4 = ctypes.c_double * MAX_DIM
 
@@ -30,13 +33,14 @@ class Model(ctypes.Structure):
         ("x0", _D4), ("a0", _D4), ("a1", _D4), ("b0", _D4), ("b1", _D4),
         ("c0", _D4), ("c1", _D4), ("j0", _D4), ("j1", _D4),
         ("chol", ctypes.c_double * (MAX_DIM * MAX_DIM)), ("jp", ctypes.c_double * 12),
+        ("ax", _D4),
     ]
 
 
 class Payoff(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32), ("log", ctypes.c_int32), ("strike", ctypes.c_double),
-        ("scale", ctypes.c_double), ("disc", ctypes.c_double),
+        ("scale", ctypes.c_double), ("disc", ctypes.c_double), ("param", ctypes.c_double),
     ]
 
 
@@ -188,6 +192,9 @@ def check(status, where):
 
 def check_layout():
     """The ctypes mirrors must match the compiled structures byte for byte."""
+    if _LIB.sdemc_abi_version() != ABI_VERSION:
+        raise RuntimeError("ABI mismatch: library version %d, ctypes mirror %d"
+                           % (_LIB.sdemc_abi_version(), ABI_VERSION))
     pairs = [("sdemc_sizeof_job", Job), ("sdemc_sizeof_model", Model),
              ("sdemc_sizeof_path_out", PathOut), ("sdemc_sizeof_fixed_in", FixedIn)]
     for fn, cls in pairs:

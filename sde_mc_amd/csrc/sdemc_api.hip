@@ -37,6 +37,8 @@ int validate_job(const sdemc_job* job) {
       return SDEMC_ERR_ARG;
     for (int j = 0; j < m.dim; ++j)
       if (!finite(m.chol[i * SDEMC_MAX_DIM + j])) return SDEMC_ERR_ARG;
+    if (!finite(m.ax[i])) return SDEMC_ERR_ARG;
+    if (m.ax[i] != 0.0 && (i == 0 || m.noise != SDEMC_NOISE_DIAG)) return SDEMC_ERR_UNSUPPORTED;
   }
   if (m.rate > 0.0) {
     if (m.jump_law == SDEMC_JUMP_LOGNORMAL) {
@@ -54,7 +56,10 @@ int validate_job(const sdemc_job* job) {
     }
   }
   const sdemc_payoff& p = job->payoff;
-  if (p.kind < SDEMC_PAYOFF_EURO_CALL || p.kind > SDEMC_PAYOFF_IDENTITY) return SDEMC_ERR_UNSUPPORTED;
+  if (p.kind < SDEMC_PAYOFF_EURO_CALL || p.kind > SDEMC_PAYOFF_RAINBOW_EVEN)
+    return SDEMC_ERR_UNSUPPORTED;
+  if (p.kind == SDEMC_PAYOFF_ASIAN_CALL && (!finite(p.param) || !(p.param > 0.0)))
+    return SDEMC_ERR_ARG;
   if (p.log != 0 && p.log != 1) return SDEMC_ERR_ARG;
   if (!finite(p.strike) || !finite(p.scale) || !finite(p.disc)) return SDEMC_ERR_ARG;
   if (!finite(job->T) || !(job->T > 0.0)) return SDEMC_ERR_ARG;

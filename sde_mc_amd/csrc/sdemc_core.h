@@ -186,11 +186,12 @@ struct KModel {
   double rate;
   float inv_rate;  // 1 / lambda (fp32 jump clock)
   float hv_kappa, hv_drift, hv_xi;  // Heston variance: kappa, kappa theta - xi^2/2, xi
+  float ax[SDEMC_MAX_DIM];          // drift of component i gains ax[i] x_0 ('diag')
 };
 
 struct KPayoff {
   int32_t kind, log;
-  float strike, scale, disc;
+  float strike, scale, disc, param;
 };
 
 // Everything one job needs in the kernels, plus derived grid constants.
@@ -261,6 +262,7 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
     f.j0[i] = (float)m.j0[i];
     f.j1[i] = (float)m.j1[i];
     for (int q = 0; q < SDEMC_MAX_DIM; ++q) f.L[i][q] = (float)m.chol[i * SDEMC_MAX_DIM + q];
+    f.ax[i] = (float)m.ax[i];
   }
   f.exact = m.exact_jumps;
   f.law = m.jump_law;
@@ -298,6 +300,7 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.pay.strike = (float)j.payoff.strike;
   k.pay.scale = (float)j.payoff.scale;
   k.pay.disc = (float)j.payoff.disc;
+  k.pay.param = (float)j.payoff.param;
   k.T = j.T;
   k.steps = j.steps;
   k.grid = j.grid;
@@ -413,6 +416,21 @@ __device__ __forceinline__ float payoff_raw(const KPayoff& p, const float (&x)[D
       v = fmaxf(m, K);
       break;
     }
+    case SDEMC_PAYOFF_ASIAN_CALL: {
+      // the running integral in component 1 over the averaging time, after
+      // the transform above (AsianCall.payoff on Option.transform's output)
+      float a = __fdiv_rn(s[D > 1 ? 1 : 0], p.param);
+      if (p.log) a = fast_exp(a);
+      v = a > K ? a - K : 0.0f;
+      break;
+    }
+    case SDEMC_PAYOFF_RAINBOW_EVEN: {
+      float m = s[0];
+#pragma unroll
+      for (int i = 2; i < D; i += 2) m = fmaxf(m, s[i]);
+      v = m > K ? m - K : 0.0f;
+      break;
+    }
     default: v = s[0]; break;
   }
   return v;
@@ -441,9 +459,11 @@ __device__ __forceinline__ void euler(const KModel& f, float (&x)[D], float dt,
     x[1] = y * y;
     return;
   }
+  const float s0 = x[0];  // pre-step x_0 for the cross drift ax[i] x_0
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const float a = __fmaf_rn(f.a1[i], x[i], f.a0[i]);
+    float a = __fmaf_rn(f.a1[i], x[i], f.a0[i]);
+    if (FAM == FAM_MERTON && i > 0) a = __fmaf_rn(f.ax[i], s0, a);
     const float b = __fmaf_rn(f.b1[i], x[i], f.b0[i]);
     const float t = __fmaf_rn(a, dt, x[i]);
     if (FAM == FAM_MERTON) {

@@ -194,16 +194,25 @@ class AsianCall(Option):
         self.time_interval = time_interval
         self.strike = strike
 
+    kernel_kind = L.PAYOFF_ASIAN_CALL
+
     def payoff(self, x):
         avg = x[:, 1] / self.time_interval
         if self.log:
             avg = avg.exp()
         return _floor_at_zero(avg - self.strike)
 
+    def kernel_payoff(self, p):
+        if not super().kernel_payoff(p):
+            return False
+        p.param = _f(self.time_interval)
+        return True
+
 
 class HestonRainbow(_Struck):
     """Call on the maximum of the price components 0, 2, 4, ... of a
     price/variance state (options.py:301-311)."""
+    kernel_kind = L.PAYOFF_RAINBOW_EVEN
 
     def payoff(self, x):
         return _floor_at_zero(x[:, 0::2].amax(dim=1) - self.strike)
@@ -228,4 +237,5 @@ class ConstantShortRate:
 
 
 # payoff implementations the kernels evaluate themselves
-_IN_KERNEL = (EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, BestOf)
+_IN_KERNEL = (EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, BestOf, AsianCall,
+              HestonRainbow)

@@ -9,7 +9,7 @@ import torch.nn as nn
 from sde_mc_amd import cv as CV
 from sde_mc_amd import mc as MC
 from sde_mc_amd.nets import Gru, Lstm, Mlp, ZeroFunction
-from sde_mc_amd.options import ConstantShortRate, EuroCall, AsianCall
+from sde_mc_amd.options import EuroCall
 from sde_mc_amd.varred import EarlyStopping, integrate_cv
 from tests import cv_emul
 
@@ -116,6 +116,13 @@ def test_unsupported_networks():
     assert MC._is_eval_packable(Mlp(2, [8], 1, batch_norm=False, batch_norm_init=False))
 
 
+class _SquaredCall(EuroCall):
+    """A payoff the kernels do not know."""
+
+    def payoff(self, x):
+        return torch.clamp(x[:, 0] - self.strike, min=0.0) ** 2
+
+
 def test_fused_eligibility_rules():
     from sde_mc_amd.problem import BlackScholesEuroCall, MertonEuroCall
     p = BlackScholesEuroCall.default_params(10, 'cpu')
@@ -123,7 +130,7 @@ def test_fused_eligibility_rules():
     # a discounter other than a constant short rate, or a payoff outside the
     # kernel's set, routes to the torch loops
     assert MC.fused_control_variates(f, p.solver, p.payoff, lambda t: torch.exp(-0.02 * t)) is None
-    assert MC.fused_control_variates(f, p.solver, AsianCall(3.0, 1.0), p.discounter) is None
+    assert MC.fused_control_variates(f, p.solver, _SquaredCall(1.0), p.discounter) is None
     assert MC.fused_control_variates(Lstm(2, 4, 1), p.solver, p.payoff, p.discounter) is None
     q = MertonEuroCall.default_params(10, 'cpu')
     g = Mlp(2, [8], 1, batch_norm=False, batch_norm_init=False)

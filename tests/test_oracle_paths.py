@@ -372,3 +372,75 @@ def test_heston_oracle_price_extrapolates_to_the_closed_form():
     se = np.std(4 * p128 - p32) / 3 / math.sqrt(P)
     assert abs(extrap - want) < 4 * se + 1e-4, (extrap, want, se)
     assert p32.mean() < p128.mean() < want + 4 * se  # the bias shrinks with h
+
+
+def _asian_job(P, N, sigma=0.3, mu=0.05, T=1.0, grid='uniform', base=None, seed=11):
+    import sde_mc_amd as sm
+    base = base or sm.Gbm(mu, sigma, torch.tensor([1.0]), 1)
+    return J.job_for(sm.AsianWrapper(base), T, N, P, grid=grid,
+                     option=sm.AsianCall(T, 1.0), seed=seed)
+
+
+def test_asian_integral_is_the_left_riemann_sum():
+    """A_N = sum_n S_n h with S_n the pre-step price (AsianWrapper's drift
+    x[:, 0] at the start of the step, sde.py:389-390)."""
+    P, N = 16, 40
+    job = _asian_job(P, N)
+    fx = J.fixed_uniform(job, seed=4)
+    _, term, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    dw = fx.a['dw'][:, 0, :].astype(np.float64)
+    S = np.vstack([np.ones((1, P)), np.cumprod(1 + 0.05 / N + 0.3 * dw, axis=0)])
+    assert np.allclose(term[0, 0], S[-1], rtol=1e-13)
+    assert np.allclose(term[0, 1], S[:-1].sum(axis=0) / N, rtol=1e-13)
+    # the post-step price would give a different sum
+    assert not np.allclose(term[0, 1], S[1:].sum(axis=0) / N, rtol=1e-6)
+
+
+def test_asian_mean_matches_the_euler_expectation():
+    """E[A_N] = S_0 h sum_{n<N} (1 + mu h)^n exactly for the Euler scheme."""
+    P, N, mu = 1 << 15, 50, 0.05
+    job = _asian_job(P, N, mu=mu)
+    pay, term, _ = oracle.simulate(job, 0, P)
+    h = 1.0 / N
+    want = h * sum((1 + mu * h) ** n for n in range(N))
+    a = term[0, 1]
+    assert abs(a.mean() - want) < 4 * a.std() / math.sqrt(P)
+    # the payoff is the call on A / T (options.py:295-298)
+    assert np.allclose(pay, np.maximum(a / 1.0 - 1.0, 0.0), atol=1e-12)
+
+
+def test_asian_over_merton_on_the_adapted_grid():
+    """E[A_T] = S_0 (e^{mu T} - 1) / mu for the compensated Merton price; the
+    adapted Euler grid is within its O(h) bias of it."""
+    P, N, T = 1 << 14, 100, 1.0
+    job = _asian_job(P, N, T=T, grid='adapted', base=J.merton_sde(mu=0.05))
+    _, term, _ = oracle.simulate(job, 0, P)
+    want = (math.exp(0.05 * T) - 1) / 0.05
+    a = term[0, 1]
+    assert abs(a.mean() - want) < 4 * a.std() / math.sqrt(P) + 2e-3
+
+
+def _torch_payoff(option, x):
+    return float(option(torch.tensor([x], dtype=torch.float64)))
+
+
+EXTRA_PAYOFFS = [
+    ('ASIAN_CALL', lambda sm: sm.AsianCall(2.0, 1.0), (1.1, 2.4)),
+    ('ASIAN_CALL', lambda sm: sm.AsianCall(2.0, 1.3), (1.1, 2.4)),
+    ('ASIAN_CALL', lambda sm: sm.AsianCall(2.0, 1.0, log=True, discount=0.9), (0.1, -0.2)),
+    ('RAINBOW_EVEN', lambda sm: sm.HestonRainbow(1.0), (1.2, 0.04)),
+    ('RAINBOW_EVEN', lambda sm: sm.HestonRainbow(1.1), (1.2, 0.04, 1.5, 0.09)),
+    ('RAINBOW_EVEN', lambda sm: sm.HestonRainbow(1.6), (1.2, 0.04, 1.5, 0.09)),
+]
+
+
+@pytest.mark.parametrize("kind,make,x", EXTRA_PAYOFFS)
+def test_oracle_asian_and_heston_rainbow_payoffs(kind, make, x):
+    """The oracle's payoff of a frozen state against the torch option (the
+    reference's composition: Option.transform, then payoff)."""
+    import sde_mc_amd as sm
+    opt = make(sm)
+    job = J.frozen_state_job(kind, opt.strike, x)
+    assert opt.kernel_payoff(job.payoff)
+    pay, _, _ = oracle.simulate(job, 0, 1)
+    assert pay[0] == pytest.approx(_torch_payoff(opt, x), rel=1e-12, abs=1e-14)
