@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/sdemc.h"
 
 // No implicit multiply-add fusion anywhere below: every fused operation is
@@ -737,13 +739,31 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
     jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
     kj = step_of(jc.tau, j.n_over_T, N);
   }
+  // Arrival-clock jumps of the coming window of 4U steps are drawn together
+  // at the window's start into a queue of JQ (step, mark) slots in arrival
+  // order, so the wavefront runs the draw code about once per window rather
+  // than once per step in which any of its lanes jumps.  A step checks only
+  // the queue's head.  A lane with more jumps in a window than slots takes the
+  // rest from the clock inside the step loop; that check is compiled only
+  // into the windows where some lane of the wavefront needs it.  Draw order,
+  // marks and their application are those of the per-step form, so results
+  // are unchanged.
+  constexpr int JQ = 2;
+  const bool clock = !pois && f.rate > 0.0;  // job-uniform
+  int32_t qk[JQ];
+  float qJ[JQ];
+#pragma unroll
+  for (int q = 0; q < JQ; ++q) {
+    qk[q] = 0x7fffffff;
+    qJ[q] = 0.0f;
+  }
 #pragma unroll
   for (int i = 0; i < D; ++i) x[i] = f.x0[i];
   if (STORE && lane_valid) {
 #pragma unroll
     for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
   }
-  auto group = [&](int32_t n0) {
+  auto group = [&](int32_t n0, auto spill) {
     float z[4 * K];
     const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
 #pragma unroll
@@ -767,7 +787,26 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
         if (pois) {
           const uint32_t w = s == 0 ? pw.x : (s == 1 ? pw.y : (s == 2 ? pw.z : pw.w));
           if (w < j.pois_T[0]) jsum = poisson_jumps<D>(j, c.k0, c.k1, plo, phi, w, n, x, xo);
-        } else if (n == kj) {
+        } else if (n == qk[0]) {
+          do {
+            if (f.exact) {
+              float y[D];
+#pragma unroll
+              for (int i = 0; i < D; ++i) y[i] = x[i];
+              add_jump(f, x, y, qJ[0]);
+            } else {
+              add_jump(f, x, xo, qJ[0]);
+            }
+            jsum += qJ[0];
+#pragma unroll
+            for (int q = 0; q + 1 < JQ; ++q) {
+              qk[q] = qk[q + 1];
+              qJ[q] = qJ[q + 1];
+            }
+            qk[JQ - 1] = 0x7fffffff;
+          } while (n == qk[0]);
+        }
+        if (decltype(spill)::value && n == kj) {
           do {
             const float J = jump_mark(f, jc.m1, jc.m2);
             if (f.exact) {
@@ -810,10 +849,34 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
     }
   };
   for (int32_t n0 = 0; n0 < N; n0 += 4 * U) {
-    group(n0);
+    bool spill = false;
+    if (clock) {
+      const int32_t wend = n0 + 4 * U;
 #pragma unroll
-    for (int u = 1; u < U; ++u)
-      if (n0 + 4 * u < N) group(n0 + 4 * u);
+      for (int q = 0; q < JQ; ++q) {
+        if (__any(kj < wend)) {
+          if (kj < wend) {
+            qk[q] = kj;
+            qJ[q] = jump_mark(f, jc.m1, jc.m2);
+            ++jc.j;
+            jc.draw(c.k0, c.k1, plo, phi, f.inv_rate);
+            kj = step_of(jc.tau, j.n_over_T, N);
+          }
+        }
+      }
+      spill = __any(kj < wend);
+    }
+    if (spill) {
+      group(n0, std::true_type{});
+#pragma unroll
+      for (int u = 1; u < U; ++u)
+        if (n0 + 4 * u < N) group(n0 + 4 * u, std::true_type{});
+    } else {
+      group(n0, std::false_type{});
+#pragma unroll
+      for (int u = 1; u < U; ++u)
+        if (n0 + 4 * u < N) group(n0 + 4 * u, std::false_type{});
+    }
   }
 }
 
