@@ -194,6 +194,9 @@ def simulate_data(trials, solver, payoff, discounter, bs=1000, inference=False):
 def simulate_adapted_data(trials, solver, payoff, discounter, bs=1000, inference=False):
     """DataLoader of AdaptedPathData on HIP-simulated jump-adapted paths (mc.py:391-399)."""
     trials = int(trials)
+    if getattr(solver, 'grid', 'adapted') != 'adapted':
+        raise NotImplementedError('jump control variates need the jump-adapted grid '
+                                  '(grid=\'adapted\'): the uniform grid keeps no pre-jump states')
     paths, aux, payoffs = _store_paths(trials, solver, payoff, discounter)
     normals, time_paths, left_paths, total, jump_paths = aux
     ds = AdaptedPathData(paths[:, :total + 1], payoffs, normals[:, :total],

@@ -282,3 +282,58 @@ def test_estimators_end_to_end():
     assert math.isfinite(st.sample_mean) and st.sample_std > 0
     tc = sm.mc_terminal_cv(1 << 16, p.solver, p.payoff, p.discounter, bs=1 << 15)
     assert math.isfinite(tc.sample_mean)
+
+
+@pytest.mark.parametrize("P", [0, 1, 63, 65])
+def test_cv_ragged_and_empty_jobs(P):
+    job = J.job_for(J.merton_sde(), 1.0, 12, P, grid='adapted', option=sm.EuroCall(1.0), seed=21)
+    f = _net(2, [20, 20], 1, seed=1)
+    g = _net(2, [20], 1, seed=2)
+    s, gam = _cv_run(job, f, g, int(job.max_steps), 0.02, 0.1)
+    assert s[2] == P and gam.shape == (P,)
+    if P:
+        st = _store(job)
+        ref, size = _adapted_reference(st, job, f, g, 0.02, 0.1)
+        assert np.abs(gam - ref).max() <= 5e-3 * (size + np.abs(ref) + 1e-6).max()
+        assert s[0] == pytest.approx(float(gam.astype(np.float64).sum()), rel=1e-9, abs=1e-12)
+    else:
+        assert s[0] == 0.0 and s[1] == 0.0
+
+
+def test_cv_steps_zero_leaves_the_payoff():
+    job = UNIFORM_CASES['gbm1']()
+    st = _store(job)
+    _, gam = _cv_run(job, _net(2, [30], 1, seed=3), None, 0, 0.02, 0.0)
+    assert np.array_equal(gam, st['payoff'])
+
+
+def test_tol_truncation_matches_the_torch_loop():
+    """mc_apply_cvs' tol (integrate_cv's remove_steps cut) in the fused
+    kernel against the torch loop on the same stored paths."""
+    p = sm.BlackScholesEuroCall.default_params(30, 'cuda')
+    f = sm.get_mlps(p, 2, 20, 'cuda')
+    f.eval()
+    tol = 0.35
+    n = 2048
+    off = p.solver.path_counter
+    dl = sm.simulate_data(n, p.solver, p.payoff, p.discounter, bs=n, inference=True)
+    (paths, normals), pay = next(iter(dl))
+    tp = sm.varred._grid(p.solver, paths.device)
+    disc = p.discounter(tp).view(1, -1, 1)
+    gt = sm.varred._diffusion_gammas(f, paths, normals, pay, tp, disc, p.solver,
+                                     tol).detach().cpu().numpy()
+    p.solver.path_counter = off
+    fused = sm.mc.fused_control_variates(f, p.solver, p.payoff, p.discounter, tol=tol)
+    assert fused.cv_steps == sm.remove_steps(tol, 30, 3.0) == 26
+    _, _, gam = fused.run(n, gamma=True)
+    err = np.abs(gam.cpu().numpy() - gt)
+    print('tol=%.2f fused vs torch: max %.3e mean %.3e' % (tol, err.max(), err.mean()), flush=True)
+    assert err.mean() < 2e-3 and err.max() < 5e-2
+
+
+def test_jump_cv_on_the_uniform_grid_is_refused():
+    q = sm.MertonEuroCall(0.02, 0.2, 1, -0.05, 0.3, 1, 1, 3, 20, 'cuda', grid='uniform')
+    fg = sm.get_mlps(q, 1, 10, 'cuda')
+    assert sm.mc.fused_control_variates(fg, q.solver, q.payoff, q.discounter) is None
+    with pytest.raises(NotImplementedError):
+        sm.mc_apply_cvs(fg, q.solver, 1000, q.payoff, q.discounter, sim_bs=1000, bs=1000)
