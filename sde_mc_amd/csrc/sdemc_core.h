@@ -1158,11 +1158,27 @@ __device__ __forceinline__ bool isclose32(float tau, float t) {
   return tau == t || err <= allowed;
 }
 
+// Philox source of the adapted grid.  The pending jump (tau1, J1) and the one
+// after it (tau2, J2) are held drawn, marks included, so a jump costs the
+// lane only a shift; refill(), called by the whole wavefront every few
+// steps, draws the replacements of all its lanes in one pass.  A lane that
+// jumps again before a refill draws in place.  Each jump j takes the same
+// Philox block and the arrival times accumulate in the same fp32 order as a
+// jump-by-jump clock, so the grid and the marks do not depend on when the
+// draws happen.
 template <int D, int FAM>
 struct AdaptRngSrc {
   NormalStream ns;
-  JumpClock jc;
+  JumpClock jc;  // the last jump drawn
   uint32_t plo, phi, k0, k1;
+  float tau1, J1, tau2, J2;
+  bool have2;
+  __device__ __forceinline__ void draw_next(const KModel& f, float& tau, float& J) {
+    ++jc.j;
+    jc.draw(k0, k1, plo, phi, f.inv_rate);
+    tau = jc.tau;
+    J = jump_mark(f, jc.m1, jc.m2);
+  }
   __device__ __forceinline__ void init(const KModel& f, const PathCtx& c) {
     k0 = c.k0;
     k1 = c.k1;
@@ -1170,18 +1186,39 @@ struct AdaptRngSrc {
     phi = (uint32_t)(c.path >> 32);
     ns.init(k0, k1, c.path);
     jc.init(k0, k1, plo, phi, f.rate, f.inv_rate);
+    tau1 = jc.tau;
+    J1 = f.rate > 0.0 ? jump_mark(f, jc.m1, jc.m2) : 0.0f;
+    tau2 = __builtin_inff();
+    J2 = 0.0f;
+    have2 = false;
   }
   __device__ __forceinline__ float z(uint32_t row, int k) {
     constexpr int K = NK<D, FAM>::K;
     return ns.get(row * K + k);
   }
-  __device__ __forceinline__ float next_time() const { return jc.tau; }
-  __device__ __forceinline__ float mark(const KModel& f) const {
-    return jump_mark(f, jc.m1, jc.m2);
-  }
+  __device__ __forceinline__ float next_time() const { return tau1; }
+  __device__ __forceinline__ float mark(const KModel&) const { return J1; }
   __device__ __forceinline__ void advance(const KModel& f) {
-    ++jc.j;
-    jc.draw(k0, k1, plo, phi, f.inv_rate);
+    if (have2) {
+      tau1 = tau2;
+      J1 = J2;
+      have2 = false;
+    } else {
+      draw_next(f, tau1, J1);
+    }
+  }
+  // whole wavefront: draw the second slot of every live lane that lacks one
+  // and whose pending jump falls before T (later ones are never used)
+  __device__ __forceinline__ void refill(const KModel& f, bool lane_live, float T32) {
+    if (f.rate > 0.0) {
+      const bool need = lane_live && !have2 && tau1 < T32;
+      if (__any(need)) {
+        if (need) {
+          draw_next(f, tau2, J2);
+          have2 = true;
+        }
+      }
+    }
   }
 };
 
@@ -1208,6 +1245,7 @@ struct AdaptFixedSrc {
     return jn < in->max_jumps ? in->jump_size_dev[(int64_t)jn * P + p] : 0.0f;
   }
   __device__ __forceinline__ void advance(const KModel&) { ++jn; }
+  __device__ __forceinline__ void refill(const KModel&, bool, float) {}
 };
 
 // Single level on the adapted grid.  Returns the number of steps taken;
@@ -1235,7 +1273,9 @@ __device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_
     }
     if (st.times) st.times[p] = 0.0f;
   }
+  int32_t iter = 0;
   while (__any(active)) {
+    if ((iter++ & 3) == 0) src.refill(f, active, T32);
     if (active && steps >= max_steps) {
       overflow = 1;
       active = false;
