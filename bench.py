@@ -382,7 +382,7 @@ def main():
     else:
         calib = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps,
                                          min(r.paths, 1 << 18), r.seed)) for r in reqs]
-        tune, _ = autotune(calib)
+        tune, _ = autotune(calib, confirm_jobs=jobs)
     runner = BatchRunner(max_jobs=len(jobs), partial_chunks=chunks, tune=tune)
 
     ref = None

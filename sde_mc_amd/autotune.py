@@ -62,8 +62,14 @@ def time_tune(runner, jobs, tune, reps=2):
     return float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
 
-def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False):
-    """Best BatchTune for batches like `jobs`; returns (tune, table of timings)."""
+def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confirm_jobs=None,
+             confirm_top=4):
+    """Best BatchTune for batches like `jobs`; returns (tune, table of timings).
+
+    With `confirm_jobs` (the real batch, when `jobs` is a cut-down copy of
+    it), the `confirm_top` fastest shapes of the sweep are timed again on
+    `confirm_jobs` and the fastest there is kept: a copy with capped path
+    counts ranks close shapes differently from the full-size batch."""
     classes = sorted(set(job_class(j) for j in jobs))
     key = '%s/classes%s' % (_device_key(), '-'.join(str(c) for c in classes))
     cache = load_cache() if use_cache else {}
@@ -80,6 +86,16 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False):
             print('autotune %r: %.3f ms' % (tune, ms), flush=True)
     runner.close()
     best = min(table, key=lambda r: r[1])
+    if confirm_jobs is not None:
+        ranked = sorted(table, key=lambda r: r[1])[:confirm_top]
+        counts = sum(chunks_of(j.n_paths) for j in confirm_jobs)
+        runner = BatchRunner(max_jobs=max(8, len(confirm_jobs)), partial_chunks=counts)
+        confirmed = [(k, time_tune(runner, confirm_jobs, BatchTune(*k), reps=1)) for k, _ in ranked]
+        runner.close()
+        if verbose:
+            for k, ms in confirmed:
+                print('autotune confirm %r: %.3f ms' % (k, ms), flush=True)
+        best = min(confirmed, key=lambda r: r[1])
     cache = load_cache()
     cache[key] = {'best': list(best[0]), 'table': table, 'seconds': time.time() - t0}
     if use_cache:

@@ -621,12 +621,12 @@ template <int D>
 __device__ __forceinline__ float poisson_jumps(const KJob& j, uint32_t k0, uint32_t k1,
                                                uint32_t plo, uint32_t phi, uint32_t w,
                                                int32_t n, float (&x)[D],
-                                               const float (&xo)[D]) {
+                                               const float (&xo)[D], int k_first = 0) {
   const KModel& f = j.m;
   float y;
   const int c = poisson_count(j, w, y);
   float jsum = 0.0f;
-  for (int k = 0; k < c; ++k) {
+  for (int k = k_first; k < c; ++k) {
     if (k > 0) y = poisson_mark_uniform(k0, k1, plo, phi, n, k);
     const float J = levy_icdf(f, y);
     if (f.exact) {
@@ -723,38 +723,43 @@ struct StoreCtx {
 // ===========================================================================
 // U: 4-step groups per iteration of the step loop (code shape only; the
 // draws and the arithmetic are the same for every U).
-template <int D, int FAM, bool STORE, int U = 1>
-__device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_valid, float (&x)[D],
-                                 const StoreCtx& st) {
+// POIS: the job's jump mode (per-step Poisson counts or the arrival clock),
+// a template parameter so each form keeps only its own state in registers;
+// path_uniform_rng below dispatches on the job.
+template <int D, int FAM, bool STORE, int U, bool POIS>
+__device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane_valid,
+                                      float (&x)[D], const StoreCtx& st) {
   constexpr int K = NK<D, FAM>::K;
   const KModel& f = j.m;
   const int32_t N = j.steps;
   const float h = j.h;
   const float sq = j.sqh;
   const uint32_t plo = (uint32_t)c.path, phi = (uint32_t)(c.path >> 32);
-  const bool pois = j.pois_mode != 0;
+  constexpr bool pois = POIS;
   JumpClock jc;
   int32_t kj = 0x7fffffff;
   if (!pois) {
     jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
     kj = step_of(jc.tau, j.n_over_T, N);
   }
-  // Arrival-clock jumps of the coming window of 4U steps are drawn together
-  // at the window's start into a queue of JQ (step, mark) slots in arrival
-  // order, so the wavefront runs the draw code about once per window rather
-  // than once per step in which any of its lanes jumps.  A step checks only
-  // the queue's head.  A lane with more jumps in a window than slots takes the
-  // rest from the clock inside the step loop; that check is compiled only
-  // into the windows where some lane of the wavefront needs it.  Draw order,
-  // marks and their application are those of the per-step form, so results
-  // are unchanged.
+  // Jump draws off the step loop.  The marks of the coming steps are drawn
+  // ahead into a queue of JQ (step, mark) slots, in the order the per-step
+  // form applies them, by passes in which every lane of the wavefront takes
+  // its next jump at once: the draw code then runs about max-over-lanes
+  // times per window instead of once per step in which any lane jumps.  A
+  // step checks only the queue's head.  Arrival-clock mode fills the queue
+  // for each window of 4U steps; per-step Poisson mode for each group of 4
+  // steps, from the group's count words.  Jumps beyond the queue are taken
+  // inside the step loop, by code compiled into the windows (groups) where
+  // some lane needs it.  Marks, order and arithmetic are those of the
+  // per-step form, so results are unchanged.
   constexpr int JQ = 2;
   const bool clock = !pois && f.rate > 0.0;  // job-uniform
   int32_t qk[JQ];
   float qJ[JQ];
 #pragma unroll
   for (int q = 0; q < JQ; ++q) {
-    qk[q] = 0x7fffffff;
+    qk[q] = -1;
     qJ[q] = 0.0f;
   }
 #pragma unroll
@@ -764,6 +769,42 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
     for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
   }
   auto group = [&](int32_t n0, auto spill) {
+    U4 pw = {0, 0, 0, 0};
+    uint32_t rem = 0u;                 // jumping steps of the group not yet started
+    int32_t cs = -1, kc = 0, cc = 0;   // step being drawn, its next mark, its count
+    bool povf = false;                 // some lane has Poisson jumps beyond the queue
+    if (pois) {
+      pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
+      const uint32_t T0 = j.pois_T[0];
+      rem = (pw.x < T0 ? 1u : 0u) | (pw.y < T0 ? 2u : 0u) | (pw.z < T0 ? 4u : 0u) |
+            (pw.w < T0 ? 8u : 0u);
+      const int32_t inside = N - n0;
+      if (inside < 4) rem &= (1u << inside) - 1u;
+#pragma unroll
+      for (int q = 0; q < JQ; ++q) {
+        const bool ev = cs >= 0 || rem != 0u;
+        if (__any(ev)) {
+          if (ev) {
+            if (cs < 0) {
+              cs = __builtin_ctz(rem);
+              rem &= rem - 1u;
+              kc = 0;
+            }
+            float y;
+            if (kc == 0) {
+              const uint32_t w = cs == 0 ? pw.x : (cs == 1 ? pw.y : (cs == 2 ? pw.z : pw.w));
+              cc = poisson_count(j, w, y);
+            } else {
+              y = poisson_mark_uniform(c.k0, c.k1, plo, phi, n0 + cs, kc);
+            }
+            qk[q] = n0 + cs;
+            qJ[q] = levy_icdf(f, y);
+            if (++kc == cc) cs = -1;
+          }
+        }
+      }
+      povf = __any(cs >= 0 || rem != 0u);
+    }
     float z[4 * K];
     const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
 #pragma unroll
@@ -772,8 +813,6 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
       box_muller(w.x, w.y, z[4 * b + 0], z[4 * b + 1]);
       box_muller(w.z, w.w, z[4 * b + 2], z[4 * b + 3]);
     }
-    U4 pw = {0, 0, 0, 0};
-    if (pois) pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int32_t n = n0 + s;
@@ -784,10 +823,7 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
         for (int i = 0; i < D; ++i) xo[i] = x[i];
         euler<D, FAM>(f, x, h, dW, dB);
         float jsum = 0.0f;
-        if (pois) {
-          const uint32_t w = s == 0 ? pw.x : (s == 1 ? pw.y : (s == 2 ? pw.z : pw.w));
-          if (w < j.pois_T[0]) jsum = poisson_jumps<D>(j, c.k0, c.k1, plo, phi, w, n, x, xo);
-        } else if (n == qk[0]) {
+        if (n == qk[0]) {
           do {
             if (f.exact) {
               float y[D];
@@ -803,8 +839,17 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
               qk[q] = qk[q + 1];
               qJ[q] = qJ[q + 1];
             }
-            qk[JQ - 1] = 0x7fffffff;
+            qk[JQ - 1] = -1;
           } while (n == qk[0]);
+        }
+        if (povf && (cs == s || ((rem >> s) & 1u))) {
+          const uint32_t w = s == 0 ? pw.x : (s == 1 ? pw.y : (s == 2 ? pw.z : pw.w));
+          jsum += poisson_jumps<D>(j, c.k0, c.k1, plo, phi, w, n, x, xo, cs == s ? kc : 0);
+          if (cs == s) {
+            cs = -1;
+          } else {
+            rem &= ~(1u << s);
+          }
         }
         if (decltype(spill)::value && n == kj) {
           do {
@@ -878,6 +923,15 @@ __device__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_vali
         if (n0 + 4 * u < N) group(n0 + 4 * u, std::false_type{});
     }
   }
+}
+
+template <int D, int FAM, bool STORE, int U = 1>
+__device__ __forceinline__ void path_uniform_rng(const KJob& j, const PathCtx& c, bool lane_valid,
+                                                 float (&x)[D], const StoreCtx& st) {
+  if (j.pois_mode != 0)
+    path_uniform_rng_mode<D, FAM, STORE, U, true>(j, c, lane_valid, x, st);
+  else
+    path_uniform_rng_mode<D, FAM, STORE, U, false>(j, c, lane_valid, x, st);
 }
 
 // Row layout of fixed increments per step: D ('diag'), D + 1 ('indep'

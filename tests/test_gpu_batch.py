@@ -108,3 +108,9 @@ def test_autotune_picks_a_candidate_and_it_prices_the_same():
     assert tune.key() in [c.key() for c in cands] and len(table) == 2
     plain = np.array([_plain(j) for j in jobs])
     assert np.array_equal(BatchRunner(max_jobs=16, tune=tune).run(jobs), plain)
+    # confirming the fastest shapes on the real batch keeps one of them
+    cands3 = cands + [BatchTune(4, 4, 2, 1, 2)]
+    tune2, _ = autotune(jobs[:3], candidates=cands3, use_cache=False, confirm_jobs=jobs,
+                        confirm_top=2)
+    assert tune2.key() in [c.key() for c in cands3]
+    assert np.array_equal(BatchRunner(max_jobs=16, tune=tune2).run(jobs), plain)
