@@ -84,7 +84,7 @@ class Result:
 
 
 class _Active:
-    __slots__ = ('req', 'job', 'slot', 'base', 'nc', 'next', 'cost', 'admitted')
+    __slots__ = ('req', 'job', 'slot', 'base', 'nc', 'next', 'cost', 'admitted', 'in_table')
 
 
 class PricingServer:
@@ -138,7 +138,6 @@ class PricingServer:
         return time.perf_counter() - self.t0
 
     def _admit(self):
-        staged = []
         while self.queue and self.free_slots:
             req = self.queue[0]
             nc = chunks_of(req.paths)
@@ -158,13 +157,21 @@ class PricingServer:
             a.job = self.job_builder(req)
             a.cost = job_cost(a.job)
             a.admitted = self._now()
+            a.in_table = False
             self.active[slot] = a
-            staged.append(a)
         self.peak_used = max(self.peak_used, self.pool.used)
-        if staged:
-            self.runner.prepare([a.job for a in staged], [a.base for a in staged],
-                                [a.slot for a in staged])
-            self.runner.upload_slots([a.slot for a in staged])
+
+    def _to_table(self, slots):
+        """Prepare and upload the job-table entries of requests entering their
+        first batched round (a request that runs alone on its plain kernel
+        never needs one)."""
+        new = [self.active[s] for s in slots if not self.active[s].in_table]
+        if new:
+            self.runner.prepare([a.job for a in new], [a.base for a in new],
+                                [a.slot for a in new])
+            self.runner.upload_slots([a.slot for a in new])
+            for a in new:
+                a.in_table = True
 
     # -- rounds ------------------------------------------------------------------
     def _plan(self):
@@ -241,6 +248,7 @@ class PricingServer:
             self.in_flight.append(('plain', (host, done, slots[0]), final, start,
                                    cost if full else 0.0))
         else:
+            self._to_table(slots)
             buf = next(b for b in self.runner.bufs
                        if b not in [f[1] for f in self.in_flight if f[0] == 'batch'])
             rp = plan_segments(slots, firsts, counts, costs, self.runner.tune.tile_chunks,
