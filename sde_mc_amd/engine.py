@@ -83,13 +83,17 @@ def make_job(model, T, steps, grid, n_paths, seed, path_offset, coarse_factor=0,
 
 
 class Workspace:
-    """Grow-only device scratch for chunk partials, reused across calls on a device."""
+    """Grow-only device scratch for chunk partials, reused across calls on one
+    (device, stream): launches on one stream run in order, so they may share
+    it, while launches on different streams may overlap and get their own.
+    A buffer that grows is freed back to torch's caching allocator, which
+    reuses it only in that stream's order."""
 
     def __init__(self):
         self.buf = {}
 
     def get(self, nbytes, device):
-        key = str(device)
+        key = (str(device), torch.cuda.current_stream(device).cuda_stream)
         b = self.buf.get(key)
         if b is None or b.numel() < nbytes:
             b = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)

@@ -242,3 +242,20 @@ def test_put_and_geometric_basket_payoffs_match_oracle():
         s, q, n = E.stats_to_host(E.simulate(job, dev))
         want = oracle.simulate(job, 0, 1)[0][0]
         assert n == 1 and abs(s - want) < 1e-6 * (1 + abs(want)), (kind, x, s, want)
+
+
+def test_plain_launches_on_two_streams_keep_their_own_scratch():
+    """Chunk partials live in a per-(device, stream) workspace: two jobs launched
+    on two streams at once give their serial results."""
+    dev = _dev()
+    jobs = [J.job_for(J.merton_sde(), 1.0, 200, (1 << 18) + 7 * k, option=EuroCall(1.0),
+                      seed=70 + k) for k in range(2)]
+    serial = [E.simulate(j, dev).cpu().numpy() for j in jobs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    out = []
+    for s, j in zip(streams, jobs):
+        with torch.cuda.stream(s):
+            out.append(E.simulate(j, dev))
+    torch.cuda.synchronize()
+    for o, r in zip(out, serial):
+        assert np.array_equal(o.cpu().numpy(), r)
