@@ -56,8 +56,14 @@ def cpu_baseline(reqs, budget_s=12.0, cap_paths=1 << 12):
         n_req += 1
         if t_total > budget_s:
             break
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": done_ps / t_total, "unit": "path-steps/s", "cores": torch.get_num_threads(),
-            "kind": "port",
+            "cpu": cpu, "kind": "port",
             "sample": "%d requests of the mix in order, paths capped at %d, reference solver "
                       "loop op for op in torch fp32 (%.1f s)" % (n_req, cap_paths, t_total)}
 
