@@ -40,7 +40,7 @@ __global__ __launch_bounds__(512) void k_rng(const KJob job, double* __restrict_
   if (!COUPLED) {
     float x[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      path_uniform_rng<D, FAM, STORE>(job, c, valid, x, st);
+      path_uniform_rng<D, FAM, STORE, STORE ? 1 : 2>(job, c, valid, x, st);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);
