@@ -8,6 +8,7 @@
 * trained networks: the control variate has mean zero and cuts the
   variance, and the estimators run end to end (fused and torch routes).
 """
+import ctypes
 import math
 
 import numpy as np
@@ -337,3 +338,46 @@ def test_jump_cv_on_the_uniform_grid_is_refused():
     assert sm.mc.fused_control_variates(fg, q.solver, q.payoff, q.discounter) is None
     with pytest.raises(NotImplementedError):
         sm.mc_apply_cvs(fg, q.solver, 1000, q.payoff, q.discounter, sim_bs=1000, bs=1000)
+
+
+def test_cv_entry_point_refuses_bad_arguments():
+    """sdemc_cv_simulate validates on the host and launches nothing on error."""
+    dev = _dev()
+    lib = L.lib()
+    f = CV.DeviceNetwork(_net(2, [16], 1, seed=5), 2, 1, dev)
+    job = UNIFORM_CASES['gbm1']()
+    stats = torch.zeros(4, dtype=torch.float64, device=dev)
+    ws = torch.empty(lib.sdemc_workspace_bytes(job.n_paths), dtype=torch.uint8, device=dev)
+
+    def call(job, spec):
+        return lib.sdemc_cv_simulate(ctypes.byref(job), ctypes.byref(spec), E.ptr(stats), None,
+                                     E.ptr(ws), ws.numel(), E.stream_ptr())
+
+    def spec(**kw):
+        s = CV.CvSpec()
+        s.f = f.c()
+        s.cv_steps = job.steps
+        s.rate = 0.02
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+    assert call(job, spec()) == L.OK
+    bad = spec()
+    bad.f.hidden_layers = 5
+    assert call(job, bad) == L.ERR_ARG
+    bad = spec()
+    bad.f.n_out = 2
+    assert call(job, bad) == L.ERR_ARG
+    bad = spec()
+    bad.f.weights_dev = f.words.data_ptr() + 2         # misaligned
+    assert call(job, bad) == L.ERR_ARG
+    assert call(job, spec(cv_steps=job.steps + 1)) == L.ERR_ARG
+    assert call(job, spec(g=f.c())) == L.ERR_ARG       # jump network on the uniform grid
+    dbl = J.job_for(sm.DoubleGbm(0.02, 0.2, 0.1, torch.tensor([1.0]), 1), 1.0, 10, 100,
+                    grid='adapted', option=sm.EuroCall(1.0))
+    f2 = CV.DeviceNetwork(_net(2, [16], 2, seed=6), 2, 2, dev)
+    s2 = spec()
+    s2.f = f2.c()
+    assert call(dbl, s2) == L.ERR_UNSUPPORTED          # 'indep' full noise on the adapted grid
+    mer = J.job_for(J.merton_sde(), 1.0, 10, 100, option=sm.EuroCall(1.0))
+    assert call(mer, spec()) == L.ERR_UNSUPPORTED      # jumps on the uniform grid
