@@ -259,3 +259,21 @@ def test_plain_launches_on_two_streams_keep_their_own_scratch():
     torch.cuda.synchronize()
     for o, r in zip(out, serial):
         assert np.array_equal(o.cpu().numpy(), r)
+
+
+@pytest.mark.parametrize("model,exact", [('merton', False), ('merton', True), ('levy', False),
+                                         ('levy', True)])
+def test_rng_paths_match_oracle_at_high_jump_rates(model, exact):
+    """Several jumps per lane and step: the kernels' jump queues overflow on
+    most windows and fall back to in-step draws; the paths still follow the
+    fp64 oracle on the same Philox counters."""
+    P = 2000
+    if model == 'merton':
+        sde, opt, N = J.merton_sde(rate=30.0), EuroCall(1.0), 20      # lambda h = 1.5
+    else:
+        sde, opt, N = J.levy_rainbow_sde(eps=0.01), Rainbow(1.0, log=True), 50  # lambda h ~ 1.9
+    job = J.job_for(sde, 1.0, N, P, option=opt, seed=881, exact_jumps=exact)
+    pay, term, _, s = _run_rng(job)
+    opay, oterm, _ = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    assert abs(s[0] / P - opay.mean()) < 1e-5 * max(1.0, abs(opay.mean()))
