@@ -989,8 +989,13 @@ __device__ __forceinline__ void fixed_step(const KJob& j, const sdemc_fixed_in& 
 // Fixed increments on the uniform grid: dw [N][K][P] already scaled and
 // correlated, counts [N][P], marks in CSR order per path.  The inputs of G
 // consecutive steps are loaded before the G dependent updates run, so a
-// wavefront keeps G rows of loads in flight instead of one.
-template <int D, int FAM>
+// wavefront keeps G rows of loads in flight instead of one.  With SYNC the
+// workgroup meets at a barrier after every 4G steps (every thread of it must
+// call this with the same N): its wavefronts then read neighbouring halves
+// of the same 128-byte count lines within a few steps of each other, while
+// the lines are still in L2, instead of fetching them again after drifting
+// apart.
+template <int D, int FAM, bool SYNC = false>
 __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int64_t p, int64_t P,
                                    float (&x)[D], int& bad) {
   constexpr int KF = NKF<D, FAM>::K;
@@ -1013,6 +1018,7 @@ __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int6
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) fixed_step<D, FAM>(j, in, x, dW[g], dB[g], cnt[g], off, bad);
+    if (SYNC && ((n / G) & 3) == 3) __syncthreads();
   }
   for (; n < N; ++n) {
     float dW[D], dB[D];

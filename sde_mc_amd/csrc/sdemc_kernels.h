@@ -83,6 +83,41 @@ __global__ __launch_bounds__(512) void k_rng(const KJob job, double* __restrict_
   chunk_store(dv, dv * dv, ovf, partials + 2 * chunk, cflags + chunk);
 }
 
+// Fixed increments, uniform grid, single level: every thread of a workgroup
+// runs the same steps (threads past the last path repeat path P - 1 and
+// discard it, none exits early) so the workgroup can keep its wavefronts in
+// step with barriers (path_uniform_fixed<..., true>).
+template <int D, int FAM>
+__global__ __launch_bounds__(256) void k_fixed_uniform(const KJob job, const sdemc_fixed_in in,
+                                                       double* __restrict__ partials,
+                                                       int32_t* __restrict__ cflags,
+                                                       const sdemc_path_out out,
+                                                       int64_t nchunks) {
+  const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t P = (int64_t)job.n_paths;
+  const int64_t p = chunk * WAVE + lane;
+  const bool valid = p < P;
+  const int64_t pp = valid ? p : P - 1;
+  const KPayoff& pf = job.pay;
+  int bad = 0;
+  float x[D];
+  path_uniform_fixed<D, FAM, true>(job, in, pp, P, x, bad);
+  if (chunk >= nchunks) return;  // whole wavefront, after the last barrier
+  float v = 0.0f;
+  if (valid) {
+    v = payoff_raw<D>(pf, x) * pf.disc;
+    if (out.terminal_dev) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) out.terminal_dev[(int64_t)i * P + p] = x[i];
+    }
+    if (out.payoff_dev) out.payoff_dev[p] = v;
+    if (out.nsteps_dev) out.nsteps_dev[p] = job.steps;
+  }
+  const double dv = valid ? (double)v : 0.0;
+  chunk_store(dv, dv * dv, (valid && bad) ? 2 : 0, partials + 2 * chunk, cflags + chunk);
+}
+
 // Plain fixed-increment kernel: the same stepping fed from host arrays.
 // Lanes past the last path read nothing.
 template <int D, int FAM, int GRID, bool COUPLED>
@@ -220,7 +255,9 @@ int launch_fixed_d(const sdemc_job& sj, const sdemc_fixed_in& in, const LaunchAr
     else SDEMC_L(SDEMC_GRID_UNIFORM, false);
   } else {
     if (!coupled) {
-      if (!adapted) SDEMC_L(SDEMC_GRID_UNIFORM, false);
+      if (!adapted)
+        hipLaunchKernelGGL((k_fixed_uniform<D, FAM>), g, b, 0, a.stream, job, in, a.partials,
+                           a.cflags, a.out, nchunks);
       else SDEMC_L(SDEMC_GRID_ADAPTED, false);
     } else {
       if (!adapted) SDEMC_L(SDEMC_GRID_UNIFORM, true);
