@@ -141,7 +141,9 @@ def equal_ci_leg(reqs, tune, dev, eps=1e-3, pilot=1 << 13, reps=3):
     per request (own Philox stream) sets the path count n = 1.2 (1.96 sd / eps)^2,
     the run_mc rule with a 20% margin for the pilot's error in sd, capped to
     [2^10, 2^22]; the same jobs are then priced
-    batched and plain (bit-identical results, so equal confidence intervals)."""
+    batched and plain (bit-identical results, so equal confidence intervals).
+    Timed twice: the main runs alone, and the whole run_mc workflow (pilots,
+    then main runs; mc.py:418-440), whose pilots are 128 small jobs."""
     import numpy as np
     import torch
     from sde_mc_amd import engine as E
@@ -149,9 +151,21 @@ def equal_ci_leg(reqs, tune, dev, eps=1e-3, pilot=1 << 13, reps=3):
     from sde_mc_amd.batch import BatchRunner, chunks_of
     pil = [W.request_job(W.Request(r.rid, r.model, r.strike, r.maturity, r.steps, pilot,
                                    r.seed ^ 0x5DEECE66D)) for r in reqs]
-    runner = BatchRunner(max_jobs=len(reqs), partial_chunks=sum(chunks_of(pilot) for _ in reqs),
-                         tune=tune)
-    st = runner.run(pil)
+    prunner = BatchRunner(max_jobs=len(reqs), partial_chunks=sum(chunks_of(pilot) for _ in reqs),
+                          tune=tune)
+    st = prunner.run(pil)
+    pref = st.copy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = prunner.run(pil)
+    tpb = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pp = [E.simulate(j, dev).cpu().numpy() for j in pil]
+    tpp = (time.perf_counter() - t0) / reps
+    assert np.array_equal(np.array(pp), pref), "plain and batched pilots differ"
+    st = pref
     sd = np.sqrt(np.maximum((st[:, 1] - st[:, 0] ** 2 / st[:, 2]) / (st[:, 2] - 1), 0.0))
     n = np.clip(np.ceil(1.2 * (1.96 * sd / eps) ** 2), 1 << 10, 1 << 22).astype(np.int64)
     sized = [W.Request(r.rid, r.model, r.strike, r.maturity, r.steps, int(k), r.seed)
@@ -181,7 +195,10 @@ def equal_ci_leg(reqs, tune, dev, eps=1e-3, pilot=1 << 13, reps=3):
         hw.append(1.96 * se)
     return {"eps": eps, "requests": len(jobs), "paths_min": int(n.min()), "paths_max": int(n.max()),
             "path_steps": ps, "batched_path_steps_s": ps / tb, "plain_path_steps_s": ps / tp,
-            "speedup_vs_plain": tp / tb, "max_half_width": float(max(hw)),
+            "speedup_vs_plain": tp / tb,
+            "pilot_ms": {"batched": tpb * 1e3, "plain": tpp * 1e3},
+            "workflow_speedup_vs_plain": (tp + tpp) / (tb + tpb),
+            "max_half_width": float(max(hw)),
             "requests_over_eps": int(sum(h > eps for h in hw)),
             "note": "requests capped at 2^22 paths may stay above eps"}
 
