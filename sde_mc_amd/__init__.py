@@ -17,7 +17,7 @@ from .options import (bs_binary_aon, bs_call, merton_call, bs_digital_call, bs_a
                       EuroCall, EuroPut, BinaryAoN, Basket, Rainbow, Digital, AsianCall,
                       HestonRainbow, BestOf, ConstantShortRate)
 from .solvers import (SdeSolver, DiffusionSolver, JumpDiffusionSolver, EulerSolver,
-                      HestonSolver, JumpEulerSolver)
+                      HestonSolver, JumpEulerSolver, Grid, UniformGrid)
 from .mc import (MCStatistics, mc_simple, find_num_trials, run_mc, mc_control_variates,
                  mc_apply_cvs, mc_adaptive_cv, mc_terminal_cv, simulate_data,
                  simulate_adapted_data, sim_train_control_variates, sample_batch_cost,

@@ -272,3 +272,37 @@ class HestonSolver(HestonScheme, DiffusionSolver):
 
 class JumpEulerSolver(EulerScheme, JumpDiffusionSolver):
     """Euler scheme on the jump-adapted grid (solvers.py:310-311)."""
+
+
+class Grid(ABC):
+    """Iterator over the time points of [start, end] (solvers.py:314-326)."""
+
+    def __init__(self, start, end):
+        self.start = start
+        self.end = end
+        self.time_interval = end - start
+        self.t = start
+
+    def __iter__(self):
+        return self
+
+    def __next__(self):
+        raise NotImplementedError
+
+
+class UniformGrid(Grid):
+    """start, start + h, ... up to end (within 1e-8), h = (end - start) / num_steps
+    (solvers.py:329-341)."""
+
+    def __init__(self, start, end, num_steps):
+        super().__init__(start, end)
+        self.num_steps = num_steps
+        self.h = (end - start) / num_steps
+        assert self.h > 1e-8
+
+    def __next__(self):
+        if self.t > self.end - 1e-8:
+            raise StopIteration
+        t = self.t
+        self.t += self.h
+        return t

@@ -61,3 +61,13 @@ def test_transform_log_and_discount():
     assert sm.HestonRainbow(0.5)(torch.tensor([[1.0, 9.0, 2.0, 9.0]])).item() == 1.5
     g = sm.Basket(1.0, average_type='geometric')(torch.tensor([[4.0, 1.0]]))
     assert torch.allclose(g, torch.tensor([1.0]))
+
+
+def test_uniform_grid_iterates_the_left_points_and_the_end():
+    from sde_mc_amd import UniformGrid
+    pts = list(UniformGrid(0.0, 3.0, 10))
+    # stops once t > end - 1e-8: ten left points, the accumulated end excluded
+    assert len(pts) == 10 and pts[0] == 0.0
+    assert max(abs(p - 0.3 * k) for k, p in enumerate(pts)) < 1e-14
+    with pytest.raises(AssertionError):
+        UniformGrid(0.0, 1e-9, 1)
