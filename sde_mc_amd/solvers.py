@@ -55,6 +55,13 @@ class SdeSolver(ABC):
         """Path storage (bs, steps + 1, dim) for callers that step by hand."""
         return torch.zeros(bs, steps + 1, self.sde.dim)
 
+    def step(self, t, x, h, corr_normals):
+        """One scheme step in torch (the scheme mixins provide it, schemes.py)."""
+        raise NotImplementedError
+
+    def solve(self, bs=1, return_normals=False):
+        raise NotImplementedError
+
     # -- kernel plumbing --------------------------------------------------
     def _grid_code(self):
         if self.grid not in ('uniform', 'adapted'):
@@ -197,6 +204,20 @@ class JumpDiffusionSolver(SdeSolver):
         self.max_jumps = E.reference_max_jumps(self._rate(), time_interval)
         self.exact_jumps = exact_jumps
         self.grid = grid
+
+    # -- the reference's torch helpers for stepping by hand (solvers.py:140-148)
+    def add_jumps(self, t, old_x, x, jumps):
+        """x plus the jump term c(t, old_x, J)."""
+        return x + self.sde.jumps(t, old_x, jumps)
+
+    def sample_jump_times(self, size):
+        """Arrival times: cumulative Exp(lambda) gaps along axis 1, torch RNG."""
+        rate = float(torch.as_tensor(self.sde.jump_rate()).sum())
+        return torch.empty(size).exponential_(rate).cumsum(dim=1)
+
+    def sample_one_jump(self, size):
+        """One mark per path, repeated over the dimensions."""
+        return self.sde.sample_jumps([size, 1], 'cpu').repeat(1, self.sde.dim)
 
     def solve(self, bs=1, return_normals=False, low_storage=False):
         """paths[:, :total_steps + 1] and (normals, time_paths, left_paths, total_steps,

@@ -71,3 +71,19 @@ def test_uniform_grid_iterates_the_left_points_and_the_end():
     assert max(abs(p - 0.3 * k) for k, p in enumerate(pts)) < 1e-14
     with pytest.raises(AssertionError):
         UniformGrid(0.0, 1e-9, 1)
+
+
+def test_jump_solver_torch_helpers():
+    """The by-hand stepping helpers of JumpDiffusionSolver (solvers.py:140-148)."""
+    import sde_mc_amd as sm
+    p = sm.MertonEuroCall.default_params(10, 'cpu')
+    s = p.solver
+    torch.manual_seed(0)
+    times = s.sample_jump_times((4000, 6, 1))
+    assert times.shape == (4000, 6, 1) and bool((times[:, 1:] >= times[:, :-1]).all())
+    gaps = torch.diff(times[..., 0], dim=1, prepend=torch.zeros(4000, 1))
+    assert abs(float(gaps.mean()) - 1.0 / float(s.sde.jump_rate())) < 0.05
+    j = s.sample_one_jump(5)
+    assert j.shape == (5, s.sde.dim) and bool((j > -1).all())
+    x = torch.ones(5, 1)
+    assert torch.allclose(s.add_jumps(0.0, x, x, j), x + j)
