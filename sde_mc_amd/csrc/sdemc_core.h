@@ -215,6 +215,7 @@ struct KJob {
   // k+1 jumps iff its uniform word w < pois_T[k]; the first mark is the
   // offset of w within its count's interval (see poisson_count)
   int32_t pois_mode;
+  int32_t clock_win;  // arrival-clock mode: steps per jump-queue fill window
   uint32_t pois_T[16];
   double pois_inv[16];  // reciprocal width of count c's interval at [c - 1]
 };
@@ -325,6 +326,16 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.magic = KJOB_MAGIC;
   k.class_key = class_key_of(m.noise, m.dim, j.grid, j.coarse_factor > 0);
   k.pois_mode = 0;
+  // A window's fill passes run when any lane of the wavefront jumps in it, so
+  // a window sized for about a quarter of a jump per path keeps the passes
+  // few while a third jump in the window (beyond the two-slot queue) stays
+  // rare; the step loop rounds it up to its own stride.
+  {
+    const double per_step = m.rate * j.T / (double)j.steps;
+    int32_t w = 128;
+    while (w > 4 && per_step * (double)w > 0.25) w >>= 1;
+    k.clock_win = w;
+  }
   for (int q = 0; q < 16; ++q) {
     k.pois_T[q] = 0;
     k.pois_inv[q] = 0.0;
@@ -342,18 +353,22 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
 //   else    : x4 = 1 - ln(mu*lda*(1-y)/cp)/mu
 // Branch selection compares the fp32 uniform with fp32 thresholds, as the
 // reference's torch.where on fp32 tensors does.
+// All four branches share one logarithm (fast_pow is exp2 of e log2 b and
+// fast_ln is ln 2 log2 v), so the function is straight-line code: the lanes
+// of a wavefront that land in different branches cost one evaluation, and
+// every value is the one the branch would compute.
 __device__ __forceinline__ float levy_icdf(const KModel& f, float y) {
-  float x;
-  if (y <= f.lv[0]) {
-    x = __fmaf_rn(fast_ln(f.lv[3] * y), f.lv[4], -1.0f);
-  } else if (y < f.lv[1]) {
-    x = -fast_pow(__fmaf_rn(f.lv[5], y, f.lv[6]), f.lv[10]);
-  } else if (y < f.lv[2]) {
-    x = fast_pow(__fmaf_rn(f.lv[7], y, f.lv[8]), f.lv[10]);
-  } else {
-    x = __fmaf_rn(-f.lv[4], fast_ln(f.lv[9] * (1.0f - y)), 1.0f);
-  }
-  return x;
+  const bool t1 = y <= f.lv[0];
+  const bool t2 = !t1 && y < f.lv[1];
+  const bool t3 = !t1 && !t2 && y < f.lv[2];
+  const float arg = t1 ? f.lv[3] * y
+                       : (t2 ? __fmaf_rn(f.lv[5], y, f.lv[6])
+                             : (t3 ? __fmaf_rn(f.lv[7], y, f.lv[8]) : f.lv[9] * (1.0f - y)));
+  const float lg = __builtin_amdgcn_logf(arg);
+  const float pw = __builtin_amdgcn_exp2f(f.lv[10] * lg);
+  const float ln = 0.693147180559945309f * lg;
+  return t1 ? __fmaf_rn(ln, f.lv[4], -1.0f)
+            : (t2 ? -pw : (t3 ? pw : __fmaf_rn(-f.lv[4], ln, 1.0f)));
 }
 
 // Jump mark from two Philox words.
@@ -748,7 +763,8 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   // its next jump at once: the draw code then runs about max-over-lanes
   // times per window instead of once per step in which any lane jumps.  A
   // step checks only the queue's head.  Arrival-clock mode fills the queue
-  // for each window of 4U steps; per-step Poisson mode for each group of 4
+  // for each window of clock_win steps (sized by the job's jump rate, rounded
+  // up to whole strides of 4U); per-step Poisson mode for each group of 4
   // steps, from the group's count words.  Jumps beyond the queue are taken
   // inside the step loop, by code compiled into the windows (groups) where
   // some lane needs it.  Marks, order and arithmetic are those of the
@@ -893,10 +909,13 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
       }
     }
   };
+  // arrival-clock fill windows: a whole number of step-loop strides
+  const int32_t win = ((j.clock_win + 4 * U - 1) / (4 * U)) * (4 * U);
+  int32_t wend = 0;
+  bool spill = false;
   for (int32_t n0 = 0; n0 < N; n0 += 4 * U) {
-    bool spill = false;
-    if (clock) {
-      const int32_t wend = n0 + 4 * U;
+    if (clock && n0 >= wend) {
+      wend = n0 + win;
 #pragma unroll
       for (int q = 0; q < JQ; ++q) {
         if (__any(kj < wend)) {
