@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDEMC_ABI_VERSION 2
+#define SDEMC_ABI_VERSION 3
 #define SDEMC_MAX_DIM 4
 #define SDEMC_CHUNK 64 /* paths per reduction chunk (one wavefront) */
 
@@ -330,12 +330,18 @@ size_t sdemc_sizeof_prepared_job(void);
 /* Kernel classes of the batched launch: a dedicated persistent kernel for
  * each of the 1-D 'diag' (Merton, GBM) and 2-D 'indep' (Levy) families on
  * either grid, single-level or coupled, and one generic kernel that runs
- * every supported job. */
+ * every supported job.  On the uniform grid at a single level each
+ * dedicated kernel holds one jump mode: the 2-D family has a kernel for
+ * per-step Poisson counts (class 4: Levy law at 1/16 or more expected jumps
+ * per step) and one for the arrival clock (class 8: every other job); the
+ * 1-D kernel (class 0) takes arrival-clock jobs and leaves 1-D per-step
+ * count jobs to the generic kernel. */
 #define SDEMC_MAX_GROUPS 8
 enum sdemc_class {
   SDEMC_CLASS_DIAG1 = 0,  /* + 2 * grid + coupled: classes 0..3 */
   SDEMC_CLASS_INDEP2 = 4, /* + 2 * grid + coupled: classes 4..7 */
-  SDEMC_CLASS_GENERIC = 8
+  SDEMC_CLASS_INDEP2_CLOCK = 8, /* 2-D 'indep', uniform grid, single level, arrival clock */
+  SDEMC_CLASS_GENERIC = 9
 };
 
 /* The dedicated class of a job, SDEMC_CLASS_GENERIC, or -1 for an invalid job. */

@@ -159,11 +159,11 @@ static void model_init(model_t* m, const sdemc_job* job) {
   m->pois = 0;
   memset(m->pT, 0, sizeof(m->pT));
   memset(m->pinv, 0, sizeof(m->pinv));
-  /* per-step Poisson counts: Levy marks on the uniform grid */
-  if (job->model.jump_law == SDEMC_JUMP_LEVY && job->grid == SDEMC_GRID_UNIFORM &&
-      job->model.rate > 0.0) {
+  /* per-step Poisson counts: Levy marks on the uniform grid from 1/16
+     expected jumps per step on, the arrival clock below */
+  if (job->model.jump_law == SDEMC_JUMP_LEVY && job->grid == SDEMC_GRID_UNIFORM) {
     double mu = job->model.rate * job->T / (double)job->steps;
-    m->pois = poisson_thresholds(mu, m->pT, m->pinv);
+    if (mu >= 0.0625) m->pois = poisson_thresholds(mu, m->pT, m->pinv);
   }
 }
 

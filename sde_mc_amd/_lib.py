@@ -14,7 +14,7 @@ CHUNK = 64
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_WORKSPACE, ERR_OVERFLOW, ERR_DEVICE = range(7)
 
-ABI_VERSION = 2   # SDEMC_ABI_VERSION these structures follow
+ABI_VERSION = 3   # SDEMC_ABI_VERSION these structures follow
 
 NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON = 0, 1, 2, 3
 JUMP_LOGNORMAL, JUMP_LEVY = 0, 1
@@ -70,7 +70,11 @@ class FixedIn(ctypes.Structure):
 
 SEGMENT_INTS = 6  # sdemc_segment: job, first, count, per_tile, tile_begin, reserved
 MAX_GROUPS = 8    # class groups per batched launch
-CLASS_GENERIC = 8  # dedicated classes are 0..7: 4 * (dim-2 family) + 2 * grid + coupled
+# dedicated classes 0..7: 4 * (dim-2 family) + 2 * grid + coupled; 8: the 2-D
+# family on the uniform grid at one level with the arrival clock (4 has per-step
+# counts); 9: the generic kernel
+CLASS_INDEP2_CLOCK = 8
+CLASS_GENERIC = 9
 
 
 class BatchDev(ctypes.Structure):

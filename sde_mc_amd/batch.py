@@ -65,7 +65,9 @@ def job_cost(job):
 def job_class(job):
     """Kernel class of a job (sdemc_job_class): 0..7 are the dedicated kernels
     of the 1-D 'diag' (0..3) and 2-D 'indep' (4..7) families, 2 * grid +
-    coupled within each; 8 is the generic kernel that runs every job."""
+    coupled within each, 8 the 2-D uniform single-level kernel for the
+    arrival clock (4 takes per-step Poisson counts); 9 is the generic kernel
+    that runs every job."""
     c = L.lib().sdemc_job_class(job)
     if c < 0:
         raise L.SdemcError(L.ERR_ARG, 'sdemc_job_class')

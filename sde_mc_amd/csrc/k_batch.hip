@@ -17,7 +17,9 @@ int validate_job(const sdemc_job* job);
 
 // One 64-path chunk of one job, RNG mode, no per-path outputs.  The result
 // equals the plain kernel's chunk: same Philox streams, same butterfly.
-template <int D, int FAM, int GRID, bool COUPLED, int U = 1>
+// MODE: the uniform single-level path's jump mode, 0 the arrival clock, 1
+// per-step Poisson counts, -1 whichever the job has.
+template <int D, int FAM, int GRID, bool COUPLED, int U = 1, int MODE = -1>
 __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, double* part,
                                            int32_t* cf) {
   const int lane = threadIdx.x & (WAVE - 1);
@@ -36,7 +38,10 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
   if (!COUPLED) {
     float x[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      path_uniform_rng<D, FAM, false, U>(job, c, valid, x, st);
+      if constexpr (MODE < 0)
+        path_uniform_rng<D, FAM, false, U>(job, c, valid, x, st);
+      else
+        path_uniform_rng_mode<D, FAM, false, U, MODE == 1>(job, c, valid, x, st);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);
@@ -61,12 +66,23 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
 }
 
 // Class of a job: dedicated kernels for the 1-D 'diag' and 2-D 'indep'
-// families (grid and level type folded in), else the generic kernel.
-inline int32_t job_class_of(int noise, int dim, int grid, bool coupled) {
+// families (grid and level type folded in), else the generic kernel.  The
+// dedicated uniform single-level kernels each hold one jump mode, so none
+// carries the other mode's registers: 1-D arrival clock (class 0), 2-D
+// per-step Poisson counts (class 4) and 2-D arrival clock (class 8); 1-D
+// per-step count jobs run in the generic kernel.
+inline int32_t job_class_of(const sdemc_job& j) {
+  const int noise = j.model.noise, dim = j.model.dim, grid = j.grid;
+  const bool coupled = j.coarse_factor > 0;
   int32_t fam = -1;
   if (noise == SDEMC_NOISE_DIAG && dim == 1) fam = 0;
   if (noise == SDEMC_NOISE_INDEP_COMMON && dim == 2) fam = 1;
   if (fam < 0) return SDEMC_CLASS_GENERIC;
+  if (grid == SDEMC_GRID_UNIFORM && !coupled) {
+    const bool pois = job_pois_mode(j);
+    if (fam == 0) return pois ? SDEMC_CLASS_GENERIC : SDEMC_CLASS_DIAG1;
+    return pois ? SDEMC_CLASS_INDEP2 : SDEMC_CLASS_INDEP2_CLOCK;
+  }
   return fam * 4 + grid * 2 + (coupled ? 1 : 0);
 }
 
@@ -86,13 +102,19 @@ template <int CLS, int U>
 __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double* part,
                                           int32_t* cf) {
   if (CLS < SDEMC_CLASS_GENERIC) {
-    constexpr int FAMI = CLS / 4, GRID = (CLS / 2) % 2;
-    constexpr bool COUPLED = (CLS % 2) != 0;
+    // class 8 is class 4's family, grid and level type in the other mode
+    constexpr int BASE = CLS == SDEMC_CLASS_INDEP2_CLOCK ? SDEMC_CLASS_INDEP2 : CLS;
+    constexpr int FAMI = BASE / 4, GRID = (BASE / 2) % 2;
+    constexpr bool COUPLED = (BASE % 2) != 0;
     constexpr int NOISE = FAMI == 0 ? SDEMC_NOISE_DIAG : SDEMC_NOISE_INDEP_COMMON;
     constexpr int D = FAMI == 0 ? 1 : 2;
     constexpr int FAM = FAMI == 0 ? FAM_MERTON : FAM_LEVY;
     if (job.class_key != ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + (COUPLED ? 1 : 0)) return false;
-    chunk_body<D, FAM, GRID, COUPLED, U>(job, chunk, part, cf);
+    constexpr int MODE = (GRID == SDEMC_GRID_UNIFORM && !COUPLED)
+                             ? (CLS == SDEMC_CLASS_INDEP2 ? 1 : 0)
+                             : -1;
+    if (MODE >= 0 && job.pois_mode != MODE) return false;
+    chunk_body<D, FAM, GRID, COUPLED, U, MODE>(job, chunk, part, cf);
     return true;
   } else {
     switch (job.class_key) {
@@ -234,7 +256,8 @@ static int cu_count_cached() {
 
 template <int CLS, int U = 1>
 static void launch_group(const sdemc_batch_dev& b, int g, unsigned grid, int wpb, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch<CLS, U>), dim3(grid), dim3(WAVE * wpb), 0, s, (const KJob*)b.jobs_dev,
+  hipLaunchKernelGGL((k_batch<CLS, U>), dim3(grid), dim3(WAVE * wpb),
+                     (size_t)wpb * sizeof(PoisLds), s, (const KJob*)b.jobs_dev,
                      b.segs_dev, b.seg_hdr_dev + 4 * g, b.partials_dev, b.chunk_flags_dev,
                      b.counter_dev + g, b.partial_chunks, b.max_jobs, b.max_segments);
 }
@@ -303,6 +326,10 @@ static void launch_class(const sdemc_batch_dev& b, int g, unsigned grid, int wpb
     case 5: launch_group<5>(b, g, grid, wpb, s); break;
     case 6: launch_group<6>(b, g, grid, wpb, s); break;
     case 7: launch_group<7>(b, g, grid, wpb, s); break;
+    case SDEMC_CLASS_INDEP2_CLOCK:
+      if (unroll == 2) launch_group<SDEMC_CLASS_INDEP2_CLOCK, 2>(b, g, grid, wpb, s);
+      else launch_group<SDEMC_CLASS_INDEP2_CLOCK>(b, g, grid, wpb, s);
+      break;
     default: launch_group<SDEMC_CLASS_GENERIC>(b, g, grid, wpb, s); break;
   }
 }
@@ -351,7 +378,7 @@ size_t sdemc_sizeof_prepared_job(void) { return sizeof(KJob); }
 
 int sdemc_job_class(const sdemc_job* job) {
   if (validate_job(job) != SDEMC_OK) return -1;
-  return job_class_of(job->model.noise, job->model.dim, job->grid, job->coarse_factor > 0);
+  return job_class_of(*job);
 }
 
 int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_base,
@@ -367,8 +394,7 @@ int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_ba
     k.chunk_base = chunk_base[i];
     memcpy((char*)prepared_out + (size_t)i * sizeof(KJob), &k, sizeof(KJob));
     if (classes)
-      classes[i] = job_class_of(jobs[i].model.noise, jobs[i].model.dim, jobs[i].grid,
-                                jobs[i].coarse_factor > 0);
+      classes[i] = job_class_of(jobs[i]);
   }
   return SDEMC_OK;
 }

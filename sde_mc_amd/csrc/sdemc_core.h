@@ -244,6 +244,13 @@ inline bool poisson_thresholds(double mu, uint32_t* T16, double* inv) {
   return tail[17] * 4294967296.0 < 0.5;
 }
 
+// Per-step Poisson counts pay one count word per step however rare the
+// jumps; the arrival clock pays a Philox block per jump and its fill passes
+// per window.  Levy-law jobs on the uniform grid take the counts from this
+// many expected jumps per step on (measured crossover near 0.07 on the 2-D
+// rainbow), the clock below it.
+constexpr double POIS_MIN_MU = 0.0625;
+
 constexpr uint32_t KJOB_MAGIC = 0x5DE3C0DEu;
 
 // Dispatch key of a prepared job: ((noise * 4 + dim - 1) * 2 + grid) * 2 + coupled.
@@ -340,10 +347,21 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
     k.pois_T[q] = 0;
     k.pois_inv[q] = 0.0;
   }
-  if (m.jump_law == SDEMC_JUMP_LEVY && j.grid == SDEMC_GRID_UNIFORM && m.rate > 0.0) {
+  if (m.jump_law == SDEMC_JUMP_LEVY && j.grid == SDEMC_GRID_UNIFORM) {
     const double mu = m.rate * j.T / (double)j.steps;
-    k.pois_mode = poisson_thresholds(mu, k.pois_T, k.pois_inv) ? 1 : 0;
+    if (mu >= POIS_MIN_MU) k.pois_mode = poisson_thresholds(mu, k.pois_T, k.pois_inv) ? 1 : 0;
   }
+}
+
+// Whether a job's uniform-grid jumps come from per-step Poisson counts (the
+// pois_mode make_kjob sets) rather than the arrival clock.
+inline bool job_pois_mode(const sdemc_job& j) {
+  if (j.model.jump_law != SDEMC_JUMP_LEVY || j.grid != SDEMC_GRID_UNIFORM) return false;
+  const double mu = j.model.rate * j.T / (double)j.steps;
+  if (!(mu >= POIS_MIN_MU)) return false;
+  uint32_t T[16];
+  double inv[16];
+  return poisson_thresholds(mu, T, inv);
 }
 
 // Inverse CDF of the Levy jump law of levy.py:22-30 on y in (0,1):
@@ -658,6 +676,134 @@ __device__ __forceinline__ float poisson_jumps(const KJob& j, uint32_t k0, uint3
 }
 
 // ---------------------------------------------------------------------------
+// Per-step Poisson jumps of one group of four uniform-grid steps, drawn by
+// the whole wavefront from a list in LDS.  Each lane posts its jumping steps
+// (count word below pois_T[0]); the lanes then take the list's items in
+// turn, so a pass of the draw code serves up to 64 jumping steps whichever
+// lanes they belong to, instead of one step per lane.  An item's first mark
+// comes from its count word, marks 2, 3, ... from the mark stream; the owner
+// reads them back at its step and applies them in the per-step form's order,
+// so results are those of drawing in the step loop.
+// ---------------------------------------------------------------------------
+constexpr int POIS_XCAP = 64;      // marks 2, 3, ... held per group
+constexpr uint32_t POIS_NOX = 0xFFu;  // item's further marks drawn by its owner
+
+struct PoisLds {
+  uint32_t v[4 * WAVE];   // item: count word in, first mark (float bits) out
+  uint8_t cnt[4 * WAVE];  // item's jump count
+  uint8_t own[4 * WAVE];  // owning lane * 4 + step within the group
+  uint8_t xb[4 * WAVE];   // slot of the item's second mark in x, or POIS_NOX
+  float x[POIS_XCAP];
+};
+
+// One PoisLds per wavefront of the workgroup, in the launch's dynamic LDS
+// (the launchers size it as waves per block * sizeof(PoisLds)).
+extern __shared__ PoisLds pois_lds_dyn[];
+
+__device__ __forceinline__ PoisLds& pois_lds() { return pois_lds_dyn[threadIdx.x / WAVE]; }
+
+// LDS written by some lanes of the wavefront, then read by others: the
+// wavefront's LDS operations complete in issue order, so only the compiler
+// has to be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exclusive prefix sum over the active lanes of v < 2^BITS, and the total.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
+  uint32_t pre = 0u, tot = 0u;
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const uint64_t m = __ballot((v >> b) & 1u);
+    pre += lanes_below(m) << b;
+    tot += (uint32_t)__builtin_popcountll(m) << b;
+  }
+  total = tot;
+  return pre;
+}
+
+// Draws the jumps of group n0 (steps n0 .. n0 + 3) for the whole wavefront.
+// jmask: the lane's jumping steps, pw: its count words.  Returns the lane's
+// first item; the item of step s is that plus the number of jumping steps
+// before s.  Must be called by every lane of the wavefront together.
+__device__ __forceinline__ uint32_t pois_draw_group(const KJob& j, uint32_t k0, uint32_t k1,
+                                                    uint32_t plo, uint32_t phi, int32_t n0,
+                                                    const U4& pw, uint32_t jmask, PoisLds& L) {
+  uint32_t total;
+  const uint32_t first = wave_prefix<3>((uint32_t)__builtin_popcount(jmask), total);
+  if (total == 0u) return first;
+  const uint32_t lane = __lane_id();
+  wave_lds_sync();  // the previous group's items have been read
+  {
+    uint32_t it = first;
+    const uint32_t w[4] = {pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if ((jmask >> s) & 1u) {
+        L.v[it] = w[s];
+        L.own[it] = (uint8_t)(lane * 4u + (uint32_t)s);
+        ++it;
+      }
+    }
+  }
+  wave_lds_sync();
+  const uint64_t act = __ballot(1);
+  const uint32_t rank = lanes_below(act);
+  const uint32_t nact = (uint32_t)__builtin_popcountll(act);
+  uint32_t xused = 0u;
+  for (uint32_t b = 0u; b < total; b += nact) {
+    const uint32_t i = b + rank;
+    const bool have = i < total;
+    const uint32_t ii = have ? i : 0u;
+    const uint32_t w = L.v[ii];
+    const uint32_t ow = L.own[ii];
+    int cnt = 0;
+    float J = 0.0f;
+    if (have) {
+      float y;
+      cnt = poisson_count(j, w, y);
+      J = levy_icdf(j.m, y);
+    }
+    const uint32_t e = have ? (uint32_t)(cnt - 1) : 0u;  // further marks, < 16
+    uint32_t xb = POIS_NOX;
+    if (__any(e != 0u)) {
+      uint32_t etot;
+      const uint32_t xo = xused + wave_prefix<4>(e, etot);
+      xused += etot;
+      const uint32_t olo = __shfl(plo, (int)(ow >> 2));
+      const uint32_t ohi = __shfl(phi, (int)(ow >> 2));
+      if (e != 0u && xo + e <= (uint32_t)POIS_XCAP) {
+        xb = xo;
+        const uint32_t n = (uint32_t)n0 + (ow & 3u);
+        U4 mw = {0, 0, 0, 0};
+        for (uint32_t q = 0u; q < e; ++q) {
+          // mark q + 2 of the step: word q mod 4 of mark-stream block q / 4
+          if ((q & 3u) == 0u) mw = philox4x32_10(n, STREAM_MARK + 16u * (q >> 2), olo, ohi, k0, k1);
+          const uint32_t sel = q & 3u;
+          const uint32_t u = sel == 0 ? mw.x : (sel == 1 ? mw.y : (sel == 2 ? mw.z : mw.w));
+          L.x[xo + q] = levy_icdf(j.m, u01_open(u));
+        }
+      }
+    }
+    if (have) {
+      L.v[ii] = __float_as_uint(J);
+      L.cnt[ii] = (uint8_t)cnt;
+      L.xb[ii] = (uint8_t)xb;
+    }
+  }
+  wave_lds_sync();
+  return first;
+}
+
+// ---------------------------------------------------------------------------
 // Chunk reduction: fp64 butterfly over the 64 lanes, lane 0 stores.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void chunk_store(double s, double q, int flag, double* partial2,
@@ -771,6 +917,7 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   // per-step form, so results are unchanged.
   constexpr int JQ = 2;
   const bool clock = !pois && f.rate > 0.0;  // job-uniform
+  PoisLds* const pl = pois ? &pois_lds() : nullptr;
   int32_t qk[JQ];
   float qJ[JQ];
 #pragma unroll
@@ -785,41 +932,16 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
   }
   auto group = [&](int32_t n0, auto spill) {
-    U4 pw = {0, 0, 0, 0};
-    uint32_t rem = 0u;                 // jumping steps of the group not yet started
-    int32_t cs = -1, kc = 0, cc = 0;   // step being drawn, its next mark, its count
-    bool povf = false;                 // some lane has Poisson jumps beyond the queue
+    uint32_t jm = 0u;     // the lane's jumping steps of the group
+    uint32_t first = 0u;  // its first item in the group's jump list
     if (pois) {
-      pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
+      const U4 pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
       const uint32_t T0 = j.pois_T[0];
-      rem = (pw.x < T0 ? 1u : 0u) | (pw.y < T0 ? 2u : 0u) | (pw.z < T0 ? 4u : 0u) |
-            (pw.w < T0 ? 8u : 0u);
+      jm = (pw.x < T0 ? 1u : 0u) | (pw.y < T0 ? 2u : 0u) | (pw.z < T0 ? 4u : 0u) |
+           (pw.w < T0 ? 8u : 0u);
       const int32_t inside = N - n0;
-      if (inside < 4) rem &= (1u << inside) - 1u;
-#pragma unroll
-      for (int q = 0; q < JQ; ++q) {
-        const bool ev = cs >= 0 || rem != 0u;
-        if (__any(ev)) {
-          if (ev) {
-            if (cs < 0) {
-              cs = __builtin_ctz(rem);
-              rem &= rem - 1u;
-              kc = 0;
-            }
-            float y;
-            if (kc == 0) {
-              const uint32_t w = cs == 0 ? pw.x : (cs == 1 ? pw.y : (cs == 2 ? pw.z : pw.w));
-              cc = poisson_count(j, w, y);
-            } else {
-              y = poisson_mark_uniform(c.k0, c.k1, plo, phi, n0 + cs, kc);
-            }
-            qk[q] = n0 + cs;
-            qJ[q] = levy_icdf(f, y);
-            if (++kc == cc) cs = -1;
-          }
-        }
-      }
-      povf = __any(cs >= 0 || rem != 0u);
+      if (inside < 4) jm &= (1u << inside) - 1u;
+      first = pois_draw_group(j, c.k0, c.k1, plo, phi, n0, pw, jm, *pl);
     }
     float z[4 * K];
     const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
@@ -858,13 +980,25 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
             qk[JQ - 1] = -1;
           } while (n == qk[0]);
         }
-        if (povf && (cs == s || ((rem >> s) & 1u))) {
-          const uint32_t w = s == 0 ? pw.x : (s == 1 ? pw.y : (s == 2 ? pw.z : pw.w));
-          jsum += poisson_jumps<D>(j, c.k0, c.k1, plo, phi, w, n, x, xo, cs == s ? kc : 0);
-          if (cs == s) {
-            cs = -1;
-          } else {
-            rem &= ~(1u << s);
+        if (pois && ((jm >> s) & 1u)) {
+          const uint32_t it = first + (uint32_t)__builtin_popcount(jm & ((1u << s) - 1u));
+          const PoisLds& L = *pl;
+          const int cnt = L.cnt[it];
+          const uint32_t xb = L.xb[it];
+          float J = __uint_as_float(L.v[it]);
+          for (int k = 0;;) {
+            if (f.exact) {
+              float y[D];
+#pragma unroll
+              for (int i = 0; i < D; ++i) y[i] = x[i];
+              add_jump(f, x, y, J);
+            } else {
+              add_jump(f, x, xo, J);
+            }
+            jsum += J;
+            if (++k >= cnt) break;
+            J = xb != POIS_NOX ? L.x[xb + (uint32_t)(k - 1)]
+                               : levy_icdf(f, poisson_mark_uniform(c.k0, c.k1, plo, phi, n, k));
           }
         }
         if (decltype(spill)::value && n == kj) {

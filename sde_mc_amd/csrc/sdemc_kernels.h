@@ -9,7 +9,9 @@ namespace sdemc {
 
 // Plain RNG-mode kernel: one wavefront per 64-path chunk.  Paths, payoff and
 // the chunk's fp64 partial sums in one pass; per-path outputs are optional.
-template <int D, int FAM, int GRID, bool COUPLED, bool STORE>
+// POIS: the uniform single-level path's jump mode (the job's pois_mode), so
+// each kernel holds the registers of one mode only.
+template <int D, int FAM, int GRID, bool COUPLED, bool STORE, bool POIS = false>
 __global__ __launch_bounds__(512) void k_rng(const KJob job, double* __restrict__ partials,
                                              int32_t* __restrict__ cflags,
                                              const sdemc_path_out out, int64_t nchunks) {
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(512) void k_rng(const KJob job, double* __restrict_
   if (!COUPLED) {
     float x[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      path_uniform_rng<D, FAM, STORE, STORE ? 1 : 2>(job, c, valid, x, st);
+      path_uniform_rng_mode<D, FAM, STORE, STORE ? 1 : 2, POIS>(job, c, valid, x, st);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);
@@ -208,19 +210,27 @@ int launch_rng_d(const sdemc_job& sj, const LaunchArgs& a) {
   const bool coupled = job.M > 0;
   const bool adapted = job.grid == SDEMC_GRID_ADAPTED;
   const dim3 g((unsigned)blocks), b((unsigned)threads);
-#define SDEMC_L(G, C, S)                                                                  \
-  hipLaunchKernelGGL((k_rng<D, FAM, G, C, S>), g, b, 0, a.stream, job, a.partials, a.cflags, \
+  // the per-wavefront jump lists of the uniform grid's per-step Poisson mode
+  const size_t lds = (size_t)a.wpb * sizeof(PoisLds);
+#define SDEMC_L(G, C, S)                                                                     \
+  hipLaunchKernelGGL((k_rng<D, FAM, G, C, S>), g, b, lds, a.stream, job, a.partials, a.cflags,    \
                      a.out, nchunks)
+#define SDEMC_LU(S)                                                                           \
+  if (job.pois_mode)                                                                          \
+    hipLaunchKernelGGL((k_rng<D, FAM, SDEMC_GRID_UNIFORM, false, S, true>), g, b, lds, a.stream, \
+                       job, a.partials, a.cflags, a.out, nchunks);                            \
+  else                                                                                        \
+    SDEMC_L(SDEMC_GRID_UNIFORM, false, S)
   if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON) {
     if (adapted) return SDEMC_ERR_UNSUPPORTED;
     if (coupled) SDEMC_L(SDEMC_GRID_UNIFORM, true, false);
-    else if (a.store) SDEMC_L(SDEMC_GRID_UNIFORM, false, true);
-    else SDEMC_L(SDEMC_GRID_UNIFORM, false, false);
+    else if (a.store) { SDEMC_LU(true); }
+    else { SDEMC_LU(false); }
   } else {
     if (!coupled) {
       if (!adapted) {
-        if (a.store) SDEMC_L(SDEMC_GRID_UNIFORM, false, true);
-        else SDEMC_L(SDEMC_GRID_UNIFORM, false, false);
+        if (a.store) { SDEMC_LU(true); }
+        else { SDEMC_LU(false); }
       } else {
         if (a.store) SDEMC_L(SDEMC_GRID_ADAPTED, false, true);
         else SDEMC_L(SDEMC_GRID_ADAPTED, false, false);
@@ -232,6 +242,7 @@ int launch_rng_d(const sdemc_job& sj, const LaunchArgs& a) {
     }
   }
 #undef SDEMC_L
+#undef SDEMC_LU
   return hipGetLastError() == hipSuccess ? SDEMC_OK : SDEMC_ERR_LAUNCH;
 }
 

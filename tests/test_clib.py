@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_layout():
     lib = L.load()
-    assert lib.sdemc_abi_version() == L.ABI_VERSION == 2
+    assert lib.sdemc_abi_version() == L.ABI_VERSION == 3
     L.check_layout()
     assert lib.sdemc_sizeof_prepared_job() % 8 == 0
 

@@ -140,6 +140,9 @@ CASES = [
     ('merton', 'uniform', 0), ('merton', 'adapted', 0), ('merton', 'uniform', 2),
     ('merton', 'adapted', 2), ('levy', 'uniform', 0), ('levy', 'adapted', 0),
     ('levy', 'uniform', 2), ('merton3', 'uniform', 0),
+    # Levy law at lambda h ~ 0.042 < 1/16: the uniform grid takes the arrival
+    # clock instead of per-step counts
+    ('levy_clock', 'uniform', 0), ('levy_clock', 'uniform', 2),
 ]
 
 
@@ -151,10 +154,15 @@ def test_rng_paths_match_oracle(model, grid, coupled):
     elif model == 'merton3':
         corr = torch.tensor([[1.0, 0.3, 0.1], [0.3, 1.0, -0.2], [0.1, -0.2, 1.0]])
         sde, opt = J.merton_sde(rate=3.0, dim=3, corr=corr), Rainbow(1.0)
+    elif model == 'levy_clock':
+        sde, opt = J.levy_rainbow_sde(eps=0.3), Rainbow(1.0, log=True, discount=math.exp(0.02))
     else:
         sde, opt = J.levy_rainbow_sde(), Rainbow(1.0, log=True, discount=math.exp(0.02))
     job = J.job_for(sde, 1.0, N, P, grid=grid, coarse_factor=coupled, option=opt,
                     disc=math.exp(-0.02), seed=777, path_offset=12345)
+    if model.startswith('levy') and grid == 'uniform':
+        # the jump mode the case is meant to cover
+        assert (oracle.step_jumps(job, 0, 0)[0] < 0) == (model == 'levy_clock')
     pay, term, ns, s = _run_rng(job)
     opay, oterm, ons = oracle.simulate(job, 0, P)
     assert _close(term, oterm) < TOL

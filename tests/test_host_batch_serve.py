@@ -70,16 +70,16 @@ def test_short_jobs_get_several_chunks_per_tile():
     assert dict(zip(plan.segs[:, 0], plan.segs[:, 3])) == {0: 4, 1: 8, 2: 8}
 
 
-def test_nine_classes_fold_into_eight_groups():
-    n = 9
+def test_all_classes_fold_into_eight_groups():
+    n = L.CLASS_GENERIC + 1  # every class, ten
     plan = plan_segments(np.arange(n), np.zeros(n), np.full(n, 4), np.arange(1.0, n + 1), 1,
                          np.arange(n))
     assert len(plan.groups) == L.MAX_GROUPS and L.CLASS_GENERIC in plan.groups
-    # the cheapest dedicated class (0, cost 1) joins the generic group
-    assert 0 not in plan.groups
+    # the cheapest dedicated classes (0 and 1, costs 1 and 2) join the generic group
+    assert 0 not in plan.groups and 1 not in plan.groups
     g = plan.groups.index(L.CLASS_GENERIC)
     s0, ns, _, _ = plan.hdr[g]
-    assert sorted(plan.segs[s0:s0 + ns, 0]) == [0, 8]
+    assert sorted(plan.segs[s0:s0 + ns, 0]) == [0, 1, L.CLASS_GENERIC]
 
 
 def test_job_cost_orders_levy_above_merton():
@@ -135,7 +135,9 @@ def test_request_jobs_validate_on_host():
         base = (ctypes.c_int64 * 1)(0)
         cs = ctypes.c_int32(-1)
         assert lib.sdemc_prepare_jobs(ctypes.byref(job), 1, base, buf, ctypes.byref(cs)) == L.OK
-        assert cs.value in (0, 4)  # 1-D Merton or 2-D Levy, single level, uniform grid
+        # 1-D Merton, or 2-D Levy with per-step counts or the arrival clock;
+        # single level, uniform grid
+        assert cs.value in (0, 4, L.CLASS_INDEP2_CLOCK)
 
 
 def test_job_classes():
@@ -143,10 +145,11 @@ def test_job_classes():
     m1 = J.job_for(J.merton_sde(), 1.0, 10, 100)
     m1a = J.job_for(J.merton_sde(), 1.0, 10, 100, grid='adapted')
     m1c = J.job_for(J.merton_sde(), 1.0, 10, 100, coarse_factor=2)
-    l2 = J.job_for(J.levy_rainbow_sde(), 1.0, 10, 100)
+    l2 = J.job_for(J.levy_rainbow_sde(), 1.0, 50, 100)  # lambda h ~ 0.5: per-step counts
     l2ac = J.job_for(J.levy_rainbow_sde(), 1.0, 10, 100, grid='adapted', coarse_factor=2)
     m3 = J.job_for(J.merton_sde(dim=3), 1.0, 10, 100)
-    assert [job_class(j) for j in (m1, m1a, m1c, l2, l2ac, m3)] == [0, 2, 1, 4, 7, 8]
+    l2k = J.job_for(J.levy_rainbow_sde(), 1.0, 800, 100)  # lambda h ~ 0.03: arrival clock
+    assert [job_class(j) for j in (m1, m1a, m1c, l2, l2ac, m3, l2k)] == [0, 2, 1, 4, 7, 9, 8]
     bad = J.job_for(J.merton_sde(), 1.0, 10, 100)
     bad.steps = 0
     with pytest.raises(L.SdemcError):

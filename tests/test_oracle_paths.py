@@ -307,6 +307,16 @@ def test_poisson_thresholds_match_scipy():
         assert inv == [1.0 / w if w else 0.0 for w in widths]
 
 
+def test_levy_jump_mode_switches_at_one_sixteenth_jump_per_step():
+    # per-step counts from lambda h = 1/16 on, the arrival clock below
+    sde = J.levy_rainbow_sde()  # lambda ~ 24.97
+    lam = J.job_for(sde, 1.0, 10, 1).model.rate
+    n_at = int(lam * 16)  # lambda / n_at just above 1/16
+    assert oracle.step_jumps(J.job_for(sde, 1.0, n_at, 1), 0, 0)[0] >= 0
+    assert oracle.step_jumps(J.job_for(sde, 1.0, n_at + 1, 1), 0, 0)[0] == -1
+    assert oracle.step_jumps(J.job_for(sde, 1.0, 200, 1, grid='adapted'), 0, 0)[0] == -1
+
+
 def test_per_step_counts_are_poisson_and_marks_follow_the_levy_law():
     # the Levy rainbow at 50 steps per year: lambda h = 24.97 / 50 ~ 0.5
     sde = J.levy_rainbow_sde()
