@@ -19,10 +19,11 @@ from .batch import BatchRunner, BatchTune, chunks_of, job_class
 # A CU has 4 SIMDs and the tile kernels fit 7 or 8 wavefronts per SIMD, so
 # the shapes swept are workgroups of 2, 4 or 8 wavefronts (128, 256 or 512
 # paths) with 3..8 resident wavefronts per SIMD in all, 1 or 2 chunks per
-# tile, and a step unroll of 1 or 2 four-step groups per loop iteration.
+# tile, a step unroll of 1 or 2 four-step groups per loop iteration, and the
+# class groups side by side or one after another.
 WAVES_PER_SIMD = (3, 4, 5, 6, 7, 8)
-CANDIDATES = [BatchTune(w, 4 * s // w, t, 1, u) for w in (2, 4, 8) for s in WAVES_PER_SIMD
-              for t in (1, 2) for u in (1, 2) if (4 * s) % w == 0]
+CANDIDATES = [BatchTune(w, 4 * s // w, t, c, u) for w in (2, 4, 8) for s in WAVES_PER_SIMD
+              for t in (1, 2) for c in (1, 0) for u in (1, 2) if (4 * s) % w == 0]
 
 
 def cache_path():
@@ -63,13 +64,15 @@ def time_tune(runner, jobs, tune, reps=2):
 
 
 def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confirm_jobs=None,
-             confirm_top=4):
+             confirm_top=8):
     """Best BatchTune for batches like `jobs`; returns (tune, table of timings).
 
     With `confirm_jobs` (the real batch, when `jobs` is a cut-down copy of
     it), the `confirm_top` fastest shapes of the sweep are timed again on
-    `confirm_jobs` and the fastest there is kept: a copy with capped path
-    counts ranks close shapes differently from the full-size batch."""
+    `confirm_jobs`, each also with its class groups run one after another
+    instead of side by side, and the fastest there is kept: a copy with
+    capped path counts ranks close shapes differently from the full-size
+    batch."""
     classes = sorted(set(job_class(j) for j in jobs))
     key = '%s/classes%s' % (_device_key(), '-'.join(str(c) for c in classes))
     cache = load_cache() if use_cache else {}
@@ -87,10 +90,14 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confi
     runner.close()
     best = min(table, key=lambda r: r[1])
     if confirm_jobs is not None:
-        ranked = sorted(table, key=lambda r: r[1])[:confirm_top]
+        ranked = [k for k, _ in sorted(table, key=lambda r: r[1])[:confirm_top]]
+        # and each of them with the class groups one after another: several
+        # groups of unequal work can finish sooner in turn, each on the whole
+        # device, than side by side, where the longest one ends alone
+        ranked += [f for f in (k[:3] + (1 - k[3],) + k[4:] for k in ranked) if f not in ranked]
         counts = sum(chunks_of(j.n_paths) for j in confirm_jobs)
         runner = BatchRunner(max_jobs=max(8, len(confirm_jobs)), partial_chunks=counts)
-        confirmed = [(k, time_tune(runner, confirm_jobs, BatchTune(*k), reps=1)) for k, _ in ranked]
+        confirmed = [(k, time_tune(runner, confirm_jobs, BatchTune(*k), reps=2)) for k in ranked]
         runner.close()
         if verbose:
             for k, ms in confirmed:
