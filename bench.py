@@ -75,8 +75,8 @@ MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2,
 # write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
 # runs.  Served mix round (k_batch group kernels, 128 requests, seed 0) and
 # the fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = 8.40e7   # HBM bytes of one mix round, copy-calibrated FETCH/WRITE_SIZE
-TRAFFIC_FIXED = 5.557e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
+TRAFFIC_MIX_ROUND = 1.327e8  # HBM bytes of one mix round, copy-calibrated FETCH/WRITE_SIZE
+TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 
 
 def _time_events(fn, reps):
