@@ -8,6 +8,7 @@ import pytest
 import torch
 
 import sde_mc_amd as sm
+from sde_mc_amd import _lib as L
 from sde_mc_amd import engine as E
 from sde_mc_amd.batch import BatchRunner, BatchTune
 from tests import jobs as J
@@ -37,7 +38,11 @@ def _plain(job):
 
 
 def test_batch_equals_plain_bitwise():
+    from sde_mc_amd.batch import job_class
     jobs = _mix()
+    # the uniform single-level classes of both jump modes are in the mix:
+    # 1-D clock, 2-D per-step counts, 2-D clock
+    assert {0, 4, L.CLASS_INDEP2_CLOCK} <= {job_class(j) for j in jobs}
     plain = np.array([_plain(j) for j in jobs])
     r = BatchRunner(max_jobs=64)
     for tune in (BatchTune(4, 2, 1), BatchTune(1, 4, 4), BatchTune(8, 1, 2),

@@ -172,6 +172,36 @@ def test_rng_paths_match_oracle(model, grid, coupled):
     assert s[2] == P
 
 
+@pytest.mark.parametrize("eps", [0.05, 0.3])
+def test_levy_stored_paths_in_both_jump_modes(eps):
+    """The path-materialising kernel in either uniform-grid jump mode
+    (per-step counts at lambda h ~ 0.26, the arrival clock at ~0.042): its
+    last stored row is the terminal state of the kernel that stores nothing,
+    bit for bit, its first the initial state, and the steps with a stored
+    jump sum occur at the Poisson rate 1 - exp(-lambda h)."""
+    P, N = 777, 96
+    job = J.job_for(J.levy_rainbow_sde(eps=eps), 1.0, N, P, option=Rainbow(1.0, log=True),
+                    seed=43)
+    assert (oracle.step_jumps(job, 0, 0)[0] < 0) == (eps == 0.3)
+    _, term, _, _ = _run_rng(job)
+    dev = _dev()
+    paths = torch.full((N + 1, 2, P), float('nan'), dtype=torch.float32, device=dev)
+    jumps = torch.zeros((N + 1, 2, P), dtype=torch.float32, device=dev)
+    out = L.PathOut()
+    out.paths_dev, out.jumps_dev = paths.data_ptr(), jumps.data_ptr()
+    print('stored levy P=%d N=%d eps=%g' % (P, N, eps), flush=True)
+    E.simulate(job, dev, out=out)
+    p, jp = paths.cpu().numpy(), jumps.cpu().numpy()
+    assert np.array_equal(p[N], term[0])
+    assert np.all(p[0] == 0.0) and np.all(np.isfinite(p))
+    # the jump sum is stored for both components
+    assert np.array_equal(jp[1:, 0], jp[1:, 1])
+    q = 1.0 - math.exp(-job.model.rate / N)
+    hits = int((jp[1:, 0] != 0.0).sum())
+    n = N * P
+    assert abs(hits - n * q) < 5 * math.sqrt(n * q * (1 - q))
+
+
 @pytest.mark.parametrize("P", [1, 63, 64, 65])
 def test_rng_ragged_path_counts(P):
     job = J.job_for(J.merton_sde(), 1.0, 10, P, option=EuroCall(1.0), seed=5)
