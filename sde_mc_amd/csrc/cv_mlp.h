@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_cv_adapted(const KJob job, const uint16
           h = fminf(h, fmaxf(T32 - t, 0.0f));
           dt = fminf(h, tau - t);
         }
-        const float sq = sqrtf(dt);
+        const float sq = __builtin_amdgcn_sqrtf(dt);
         float zz[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) zz[k] = src.z((uint32_t)steps, k);

@@ -1409,6 +1409,18 @@ struct AdaptRngSrc {
     constexpr int K = NK<D, FAM>::K;
     return ns.get(row * K + k);
   }
+  // normals of rows row0 .. row0 + 3 (row0 a multiple of 4): K whole blocks,
+  // zq[s K + k] = z(row0 + s, k)
+  __device__ __forceinline__ void rows4(uint32_t row0, bool, float (&zq)[4 * NK<D, FAM>::K]) {
+    constexpr int K = NK<D, FAM>::K;
+    const uint32_t b0 = (row0 >> 2) * (uint32_t)K;
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+      const U4 w = philox4x32_10(b0 + b, STREAM_BM, plo, phi, k0, k1);
+      box_muller(w.x, w.y, zq[4 * b + 0], zq[4 * b + 1]);
+      box_muller(w.z, w.w, zq[4 * b + 2], zq[4 * b + 3]);
+    }
+  }
   __device__ __forceinline__ float next_time() const { return tau1; }
   __device__ __forceinline__ float mark(const KModel&) const { return J1; }
   __device__ __forceinline__ void advance(const KModel& f) {
@@ -1451,6 +1463,15 @@ struct AdaptFixedSrc {
     if ((int32_t)row >= in->n_rows) return 0.0f;
     return in->dw_dev[((int64_t)row * K + k) * P + p];
   }
+  // rows row0 .. row0 + 3 of a live lane (a lane past the last path, or one
+  // whose path has ended, reads nothing)
+  __device__ __forceinline__ void rows4(uint32_t row0, bool live, float (&zq)[4 * NK<D, FAM>::K]) {
+    constexpr int K = NK<D, FAM>::K;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < K; ++k) zq[q * K + k] = live ? z(row0 + (uint32_t)q, k) : 0.0f;
+  }
   __device__ __forceinline__ float next_time() const {
     return jn < in->max_jumps ? in->jump_time_dev[(int64_t)jn * P + p] : __builtin_inff();
   }
@@ -1486,80 +1507,88 @@ __device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_
     }
     if (st.times) st.times[p] = 0.0f;
   }
-  int32_t iter = 0;
-  while (__any(active)) {
-    if ((iter++ & 3) == 0) src.refill(f, active, T32);
-    if (active && steps >= max_steps) {
-      overflow = 1;
-      active = false;
-    }
-    if (active) {
-      float dt, tn;
-      {
-#pragma clang fp contract(off)
-        h = fminf(h, fmaxf(T32 - t, 0.0f));
-        dt = fminf(h, tau - t);
+  // A live lane takes one step per iteration, so at the start of iteration i
+  // every live lane is at row i: the normals of rows i .. i + 3 are drawn for
+  // the wavefront together every 4 iterations, in the stream's layout (row r,
+  // normal k is index r K + k), with the jump slots' refill.
+  for (uint32_t row0 = 0; __any(active); row0 += 4) {
+    src.refill(f, active, T32);
+    float zq[4 * K];
+    src.rows4(row0, active, zq);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (active && steps >= max_steps) {
+        overflow = 1;
+        active = false;
       }
-      const float sq = sqrtf(dt);
-      float zz[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) zz[k] = src.z((uint32_t)steps, k);
-      float dW[D], dB[D], xo[D];
-      correlate<D, FAM>(f, zz, sq, dW, dB);
-#pragma unroll
-      for (int i = 0; i < D; ++i) xo[i] = x[i];
-      euler<D, FAM>(f, x, dt, dW, dB);
-      {
+      if (active) {
+        float dt, tn;
+        {
 #pragma clang fp contract(off)
-        tn = t + dt;
-      }
-      t = tn;
-      ++steps;
-      if (STORE) {
-        const int64_t r = (int64_t)steps * D;
-        if (st.left) {
-#pragma unroll
-          for (int i = 0; i < D; ++i) st.left[(r + i) * st.P + p] = x[i];
+          h = fminf(h, fmaxf(T32 - t, 0.0f));
+          dt = fminf(h, tau - t);
         }
-        if (st.times) st.times[(int64_t)steps * st.P + p] = t;
-        if (st.normals) {
-          const int64_t rn = (int64_t)(steps - 1) * D * (DiagNoise<FAM>::value ? 1 : 2);
-          if (DiagNoise<FAM>::value) {
+        const float sq = __builtin_amdgcn_sqrtf(dt);
+        float zz[K];
 #pragma unroll
-            for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + p] = dW[i];
-          } else {
+        for (int k = 0; k < K; ++k) zz[k] = zq[q * K + k];
+        float dW[D], dB[D], xo[D];
+        correlate<D, FAM>(f, zz, sq, dW, dB);
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-              st.normals[(rn + 2 * i) * st.P + p] = dW[i];
-              st.normals[(rn + 2 * i + 1) * st.P + p] = dB[i];
+        for (int i = 0; i < D; ++i) xo[i] = x[i];
+        euler<D, FAM>(f, x, dt, dW, dB);
+        {
+#pragma clang fp contract(off)
+          tn = t + dt;
+        }
+        t = tn;
+        ++steps;
+        if (STORE) {
+          const int64_t r = (int64_t)steps * D;
+          if (st.left) {
+#pragma unroll
+            for (int i = 0; i < D; ++i) st.left[(r + i) * st.P + p] = x[i];
+          }
+          if (st.times) st.times[(int64_t)steps * st.P + p] = t;
+          if (st.normals) {
+            const int64_t rn = (int64_t)(steps - 1) * D * (DiagNoise<FAM>::value ? 1 : 2);
+            if (DiagNoise<FAM>::value) {
+#pragma unroll
+              for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + p] = dW[i];
+            } else {
+#pragma unroll
+              for (int i = 0; i < D; ++i) {
+                st.normals[(rn + 2 * i) * st.P + p] = dW[i];
+                st.normals[(rn + 2 * i + 1) * st.P + p] = dB[i];
+              }
             }
           }
         }
-      }
-      float Jap = 0.0f;
-      if (isclose32(tau, t)) {
-        const float J = src.mark(f);
-        if (f.exact) {
-          float y[D];
+        float Jap = 0.0f;
+        if (isclose32(tau, t)) {
+          const float J = src.mark(f);
+          if (f.exact) {
+            float y[D];
 #pragma unroll
-          for (int i = 0; i < D; ++i) y[i] = x[i];
-          add_jump(f, x, y, J);
-        } else {
-          add_jump(f, x, xo, J);
+            for (int i = 0; i < D; ++i) y[i] = x[i];
+            add_jump(f, x, y, J);
+          } else {
+            add_jump(f, x, xo, J);
+          }
+          Jap = J;
+          src.advance(f);
+          tau = src.next_time();
         }
-        Jap = J;
-        src.advance(f);
-        tau = src.next_time();
-      }
-      if (STORE) {
-        const int64_t r = (int64_t)steps * D;
+        if (STORE) {
+          const int64_t r = (int64_t)steps * D;
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-          st.paths[(r + i) * st.P + p] = x[i];
-          if (st.jumps) st.jumps[(r + i) * st.P + p] = Jap;
+          for (int i = 0; i < D; ++i) {
+            st.paths[(r + i) * st.P + p] = x[i];
+            if (st.jumps) st.jumps[(r + i) * st.P + p] = Jap;
+          }
         }
+        active = t < T32;
       }
-      active = t < T32;
     }
   }
   if (STORE && lane_valid) {
@@ -1626,7 +1655,7 @@ __device__ int32_t path_adapted_coupled(const KJob& j, SRC& src, bool lane_valid
           hf = fminf(hf, fmaxf(T32 - tf, 0.0f));
           dt = fminf(hf, tau - tf);
         }
-        const float sq = sqrtf(dt);
+        const float sq = __builtin_amdgcn_sqrtf(dt);
         float zz[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) zz[k] = src.z((uint32_t)(it * M + s), k);
