@@ -51,9 +51,7 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
   } else {
     float xf[D], xc[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      UniRngSrc<D, FAM> src;
-      src.init(job, c);
-      path_uniform_coupled<D, FAM>(job, src, xf, xc);
+      path_uniform_rng_coupled<D, FAM>(job, c, xf, xc);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);

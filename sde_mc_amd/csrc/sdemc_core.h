@@ -877,6 +877,60 @@ struct StoreCtx {
   int32_t rows;  // S
 };
 
+// Coarse half of a coupled uniform pair (an MLMC level): the sums of the M
+// fine increments and the fine jumps of the coarse step -- their marks
+// summed, and under the post-step rule composed into one affine map per
+// component -- in the order the fine steps take them.
+template <int D>
+struct CoarseAcc {
+  float sW[D], sB[D], A[D], Bc[D];
+  float Jsum;
+  __device__ __forceinline__ void reset() {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      sW[i] = 0.0f;
+      sB[i] = 0.0f;
+      A[i] = 1.0f;
+      Bc[i] = 0.0f;
+    }
+    Jsum = 0.0f;
+  }
+  __device__ __forceinline__ void increment(const float (&dW)[D], const float (&dB)[D]) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      sW[i] += dW[i];
+      sB[i] += dB[i];
+    }
+  }
+  __device__ __forceinline__ void jump(const KModel& f, float J) {
+    if (f.exact) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const float g = __fmaf_rn(f.j1[i], J, 1.0f);
+        A[i] = A[i] * g;
+        Bc[i] = __fmaf_rn(Bc[i], g, f.j0[i] * J);
+      }
+    }
+    Jsum += J;
+  }
+  // the coarse step of size hc of the coarse state xc[0 .. D-1]
+  template <int FAM>
+  __device__ __forceinline__ void step(const KModel& f, float hc, float* xc) {
+    float y[D], yo[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) y[i] = yo[i] = xc[i];
+    euler<D, FAM>(f, y, hc, sW, sB);
+    if (f.exact) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) y[i] = __fmaf_rn(A[i], y[i], Bc[i]);
+    } else if (Jsum != 0.0f) {
+      add_jump(f, y, yo, Jsum);
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) xc[i] = y[i];
+  }
+};
+
 // ===========================================================================
 // Uniform grid, single level: N steps of h = fp32(T/N).  The 4K normals of
 // four steps come from K Philox blocks generated together, so every index
@@ -887,9 +941,12 @@ struct StoreCtx {
 // POIS: the job's jump mode (per-step Poisson counts or the arrival clock),
 // a template parameter so each form keeps only its own state in registers;
 // path_uniform_rng below dispatches on the job.
-template <int D, int FAM, bool STORE, int U, bool POIS>
+// COUPLED: x is the fine path of an MLMC pair and xc (then non-null) its
+// coarse path, stepped every M = j.M fine steps by CoarseAcc.
+template <int D, int FAM, bool STORE, int U, bool POIS, bool COUPLED = false>
 __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane_valid,
-                                      float (&x)[D], const StoreCtx& st) {
+                                      float (&x)[D], const StoreCtx& st,
+                                      float* xc = nullptr) {
   constexpr int K = NK<D, FAM>::K;
   const KModel& f = j.m;
   const int32_t N = j.steps;
@@ -930,6 +987,13 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   if (STORE && lane_valid) {
 #pragma unroll
     for (int i = 0; i < D; ++i) st.paths[(int64_t)i * st.P + c.p] = x[i];
+  }
+  CoarseAcc<D> acc;
+  int32_t cm = 0;  // fine steps taken in the current coarse step
+  if constexpr (COUPLED) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) xc[i] = f.x0[i];
+    acc.reset();
   }
   auto group = [&](int32_t n0, auto spill) {
     uint32_t jm = 0u;     // the lane's jumping steps of the group
@@ -972,6 +1036,7 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
               add_jump(f, x, xo, qJ[0]);
             }
             jsum += qJ[0];
+            if constexpr (COUPLED) acc.jump(f, qJ[0]);
 #pragma unroll
             for (int q = 0; q + 1 < JQ; ++q) {
               qk[q] = qk[q + 1];
@@ -996,6 +1061,7 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
               add_jump(f, x, xo, J);
             }
             jsum += J;
+            if constexpr (COUPLED) acc.jump(f, J);
             if (++k >= cnt) break;
             J = xb != POIS_NOX ? L.x[xb + (uint32_t)(k - 1)]
                                : levy_icdf(f, poisson_mark_uniform(c.k0, c.k1, plo, phi, n, k));
@@ -1013,10 +1079,19 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
               add_jump(f, x, xo, J);
             }
             jsum += J;
+            if constexpr (COUPLED) acc.jump(f, J);
             ++jc.j;
             jc.draw(c.k0, c.k1, plo, phi, f.inv_rate);
             kj = step_of(jc.tau, j.n_over_T, N);
           } while (kj == n);
+        }
+        if constexpr (COUPLED) {
+          acc.increment(dW, dB);
+          if (++cm == j.M) {
+            acc.template step<FAM>(f, j.hc, xc);
+            acc.reset();
+            cm = 0;
+          }
         }
         if (STORE && lane_valid) {
           const int64_t r = (int64_t)(n + 1) * D;
@@ -1085,6 +1160,21 @@ __device__ __forceinline__ void path_uniform_rng(const KJob& j, const PathCtx& c
     path_uniform_rng_mode<D, FAM, STORE, U, true>(j, c, lane_valid, x, st);
   else
     path_uniform_rng_mode<D, FAM, STORE, U, false>(j, c, lane_valid, x, st);
+}
+
+// Coupled fine/coarse pair on the uniform grid (an MLMC level), Philox
+// inputs: the single-level step loop with the coarse half riding along, so
+// it shares the 4-step normal blocks, the jump queue and the per-wavefront
+// jump lists; the draws, their order and the arithmetic are those of
+// path_uniform_coupled.
+template <int D, int FAM, int U = 1>
+__device__ __forceinline__ void path_uniform_rng_coupled(const KJob& j, const PathCtx& c,
+                                                         float (&xf)[D], float (&xc)[D]) {
+  const StoreCtx st{};
+  if (j.pois_mode != 0)
+    path_uniform_rng_mode<D, FAM, false, U, true, true>(j, c, true, xf, st, xc);
+  else
+    path_uniform_rng_mode<D, FAM, false, U, false, true>(j, c, true, xf, st, xc);
 }
 
 // Row layout of fixed increments per step: D ('diag'), D + 1 ('indep'
@@ -1186,7 +1276,9 @@ __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int6
 // h = fp32(T/Nf), coarse steps of hc = fp32(M T/Nf) whose increment is the
 // sum of its M fine ones; a jump in fine step n belongs to coarse step n / M.
 // Coarse jumps under the post-step rule compose into one affine map per
-// component, so any number of them in a coarse step is exact.
+// component, so any number of them in a coarse step is exact.  This form reads
+// fixed increments (UniFixedSrc); Philox inputs go through
+// path_uniform_rng_coupled, which takes the same steps in the same order.
 // ===========================================================================
 template <int D, int FAM, class SRC>
 __device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], float (&xc)[D]) {
@@ -1250,78 +1342,6 @@ __device__ void path_uniform_coupled(const KJob& j, SRC& src, float (&xf)[D], fl
     }
   }
 }
-
-// RNG source for the uniform coupled path: fine-step normals, and jumps
-// either from the arrival clock or from per-step Poisson counts.
-template <int D, int FAM>
-struct UniRngSrc {
-  NormalStream ns;
-  JumpClock jc;
-  uint32_t plo, phi, k0, k1;
-  int32_t kj;
-  // per-step Poisson mode: cached count block, current word and mark index
-  uint32_t pblk, pw0, pw1, pw2, pw3, w;
-  int32_t cur_n, k, cnt;
-  float y0;
-  __device__ __forceinline__ void init(const KJob& j, const PathCtx& c) {
-    k0 = c.k0;
-    k1 = c.k1;
-    plo = (uint32_t)c.path;
-    phi = (uint32_t)(c.path >> 32);
-    ns.init(k0, k1, c.path);
-    kj = 0x7fffffff;
-    pblk = 0xffffffffu;
-    cur_n = -1;
-    k = 0;
-    cnt = 0;
-    y0 = 0.5f;
-    w = 0xffffffffu;
-    if (!j.pois_mode) {
-      jc.init(k0, k1, plo, phi, j.m.rate, j.m.inv_rate);
-      kj = step_of(jc.tau, j.n_over_T, j.steps);
-    }
-  }
-  __device__ __forceinline__ void incr(const KJob& j, int32_t n, float (&dW)[D], float (&dB)[D]) {
-    constexpr int K = NK<D, FAM>::K;
-    float z[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) z[q] = ns.get((uint32_t)n * K + q);
-    correlate<D, FAM>(j.m, z, j.sqh, dW, dB);
-    if (j.pois_mode) {
-      const uint32_t b = (uint32_t)n >> 2;
-      if (b != pblk) {
-        pblk = b;
-        const U4 pw = philox4x32_10(b, STREAM_POIS, plo, phi, k0, k1);
-        pw0 = pw.x;
-        pw1 = pw.y;
-        pw2 = pw.z;
-        pw3 = pw.w;
-      }
-      const uint32_t sel = (uint32_t)n & 3u;
-      w = sel == 0 ? pw0 : (sel == 1 ? pw1 : (sel == 2 ? pw2 : pw3));
-      cur_n = n;
-      k = 0;
-      cnt = 0;
-      if (w < j.pois_T[0]) cnt = poisson_count(j, w, y0);
-    }
-  }
-  __device__ __forceinline__ bool jump_pending(const KJob& j, int32_t n) const {
-    if (j.pois_mode) return n == cur_n && k < cnt;
-    return kj == n;
-  }
-  __device__ __forceinline__ float take_jump(const KJob& j) {
-    if (j.pois_mode) {
-      const float y = k == 0 ? y0 : poisson_mark_uniform(k0, k1, plo, phi, cur_n, k);
-      ++k;
-      return levy_icdf(j.m, y);
-    }
-    const float J = jump_mark(j.m, jc.m1, jc.m2);
-    ++jc.j;
-    jc.draw(k0, k1, plo, phi, j.m.inv_rate);
-    kj = step_of(jc.tau, j.n_over_T, j.steps);
-    return J;
-  }
-};
 
 // Fixed source for the uniform coupled path (fine-grid arrays).
 template <int D, int FAM>

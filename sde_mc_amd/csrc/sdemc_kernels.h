@@ -58,9 +58,7 @@ __global__ __launch_bounds__(512) void k_rng(const KJob job, double* __restrict_
   } else {
     float xf[D], xc[D];
     if (GRID == SDEMC_GRID_UNIFORM) {
-      UniRngSrc<D, FAM> src;
-      src.init(job, c);
-      path_uniform_coupled<D, FAM>(job, src, xf, xc);
+      path_uniform_rng_coupled<D, FAM>(job, c, xf, xc);
     } else {
       AdaptRngSrc<D, FAM> src;
       src.init(job.m, c);
