@@ -1655,7 +1655,11 @@ __device__ int32_t path_adapted_coupled(const KJob& j, SRC& src, bool lane_valid
   bool active = lane_valid && (tf < T32);
   int32_t it = 0;
   overflow = 0;
+  int32_t iter = 0;
   while (__any(active)) {
+    // second jump slots for the wavefront together, every 4 iterations (the
+    // jumps keep their Philox blocks, so the marks are those drawn one by one)
+    if ((iter++ & 3) == 0) src.refill(f, active, T32);
     if (active && it >= max_steps) {
       overflow = 1;
       active = false;
