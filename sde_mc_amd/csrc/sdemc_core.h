@@ -678,23 +678,26 @@ __device__ __forceinline__ float poisson_jumps(const KJob& j, uint32_t k0, uint3
 }
 
 // ---------------------------------------------------------------------------
-// Per-step Poisson jumps of one group of four uniform-grid steps, drawn by
+// Per-step Poisson jumps of a window of eight uniform-grid steps, drawn by
 // the whole wavefront from a list in LDS.  Each lane posts its jumping steps
 // (count word below pois_T[0]); the lanes then take the list's items in
 // turn, so a pass of the draw code serves up to 64 jumping steps whichever
 // lanes they belong to, instead of one step per lane.  An item's first mark
 // comes from its count word, marks 2, 3, ... from the mark stream; the owner
 // reads them back at its step and applies them in the per-step form's order,
-// so results are those of drawing in the step loop.
+// so results are those of drawing in the step loop.  Eight steps per window
+// (two 4-word count blocks) fill the passes better than four and halve the
+// fixed cost of the list per step.
 // ---------------------------------------------------------------------------
-constexpr int POIS_XCAP = 64;      // marks 2, 3, ... held per group
+constexpr int POIS_WIN = 8;         // steps per window
+constexpr int POIS_XCAP = 128;      // marks 2, 3, ... held per window
 constexpr uint32_t POIS_NOX = 0xFFu;  // item's further marks drawn by its owner
 
 struct PoisLds {
-  uint32_t v[4 * WAVE];   // item: count word in, first mark (float bits) out
-  uint8_t cnt[4 * WAVE];  // item's jump count
-  uint8_t own[4 * WAVE];  // owning lane * 4 + step within the group
-  uint8_t xb[4 * WAVE];   // slot of the item's second mark in x, or POIS_NOX
+  uint32_t v[POIS_WIN * WAVE];   // item: count word in, first mark (float bits) out
+  uint16_t own[POIS_WIN * WAVE];  // owning lane * POIS_WIN + step within the window
+  uint8_t cnt[POIS_WIN * WAVE];  // item's jump count
+  uint8_t xb[POIS_WIN * WAVE];   // slot of the item's second mark in x, or POIS_NOX
   float x[POIS_XCAP];
 };
 
@@ -732,26 +735,26 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
   return pre;
 }
 
-// Draws the jumps of group n0 (steps n0 .. n0 + 3) for the whole wavefront.
-// jmask: the lane's jumping steps, pw: its count words.  Returns the lane's
+// Draws the jumps of window n0 (steps n0 .. n0 + 7) for the whole wavefront.
+// jmask: the lane's jumping steps, w: its count words.  Returns the lane's
 // first item; the item of step s is that plus the number of jumping steps
 // before s.  Must be called by every lane of the wavefront together.
-__device__ __forceinline__ uint32_t pois_draw_group(const KJob& j, uint32_t k0, uint32_t k1,
-                                                    uint32_t plo, uint32_t phi, int32_t n0,
-                                                    const U4& pw, uint32_t jmask, PoisLds& L) {
+__device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0, uint32_t k1,
+                                                     uint32_t plo, uint32_t phi, int32_t n0,
+                                                     const uint32_t (&w)[POIS_WIN],
+                                                     uint32_t jmask, PoisLds& L) {
   uint32_t total;
-  const uint32_t first = wave_prefix<3>((uint32_t)__builtin_popcount(jmask), total);
+  const uint32_t first = wave_prefix<4>((uint32_t)__builtin_popcount(jmask), total);
   if (total == 0u) return first;
   const uint32_t lane = __lane_id();
-  wave_lds_sync();  // the previous group's items have been read
+  wave_lds_sync();  // the previous window's items have been read
   {
     uint32_t it = first;
-    const uint32_t w[4] = {pw.x, pw.y, pw.z, pw.w};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < POIS_WIN; ++s) {
       if ((jmask >> s) & 1u) {
         L.v[it] = w[s];
-        L.own[it] = (uint8_t)(lane * 4u + (uint32_t)s);
+        L.own[it] = (uint16_t)(lane * (uint32_t)POIS_WIN + (uint32_t)s);
         ++it;
       }
     }
@@ -765,13 +768,13 @@ __device__ __forceinline__ uint32_t pois_draw_group(const KJob& j, uint32_t k0, 
     const uint32_t i = b + rank;
     const bool have = i < total;
     const uint32_t ii = have ? i : 0u;
-    const uint32_t w = L.v[ii];
+    const uint32_t wi = L.v[ii];
     const uint32_t ow = L.own[ii];
     int cnt = 0;
     float J = 0.0f;
     if (have) {
       float y;
-      cnt = poisson_count(j, w, y);
+      cnt = poisson_count(j, wi, y);
       J = levy_icdf(j.m, y);
     }
     const uint32_t e = have ? (uint32_t)(cnt - 1) : 0u;  // further marks, < 16
@@ -780,11 +783,11 @@ __device__ __forceinline__ uint32_t pois_draw_group(const KJob& j, uint32_t k0, 
       uint32_t etot;
       const uint32_t xo = xused + wave_prefix<4>(e, etot);
       xused += etot;
-      const uint32_t olo = __shfl(plo, (int)(ow >> 2));
-      const uint32_t ohi = __shfl(phi, (int)(ow >> 2));
+      const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)POIS_WIN));
+      const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)POIS_WIN));
       if (e != 0u && xo + e <= (uint32_t)POIS_XCAP) {
         xb = xo;
-        const uint32_t n = (uint32_t)n0 + (ow & 3u);
+        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)POIS_WIN;
         U4 mw = {0, 0, 0, 0};
         for (uint32_t q = 0u; q < e; ++q) {
           // mark q + 2 of the step: word q mod 4 of mark-stream block q / 4
@@ -997,17 +1000,36 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     for (int i = 0; i < D; ++i) xc[i] = f.x0[i];
     acc.reset();
   }
+  uint32_t wjm = 0u;     // per-step Poisson mode: the lane's jumping steps of the window
+  uint32_t wfirst = 0u;  // and its first item in the window's jump list
   auto group = [&](int32_t n0, auto spill) {
     uint32_t jm = 0u;     // the lane's jumping steps of the group
-    uint32_t first = 0u;  // its first item in the group's jump list
+    uint32_t first = 0u;  // its first item in the window's jump list
     if (pois) {
-      const U4 pw = philox4x32_10((uint32_t)(n0 >> 2), STREAM_POIS, plo, phi, c.k0, c.k1);
-      const uint32_t T0 = j.pois_T[0];
-      jm = (pw.x < T0 ? 1u : 0u) | (pw.y < T0 ? 2u : 0u) | (pw.z < T0 ? 4u : 0u) |
-           (pw.w < T0 ? 8u : 0u);
-      const int32_t inside = N - n0;
-      if (inside < 4) jm &= (1u << inside) - 1u;
-      first = pois_draw_group(j, c.k0, c.k1, plo, phi, n0, pw, jm, *pl);
+      if ((n0 & (POIS_WIN - 1)) == 0) {
+        // a new window: the count words of its steps (4 per Philox block)
+        uint32_t w[POIS_WIN];
+        const uint32_t T0 = j.pois_T[0];
+        wjm = 0u;
+#pragma unroll
+        for (int g = 0; g < POIS_WIN / 4; ++g) {
+          U4 pw = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+          if (n0 + 4 * g < N)
+            pw = philox4x32_10((uint32_t)(n0 >> 2) + (uint32_t)g, STREAM_POIS, plo, phi, c.k0, c.k1);
+          w[4 * g + 0] = pw.x;
+          w[4 * g + 1] = pw.y;
+          w[4 * g + 2] = pw.z;
+          w[4 * g + 3] = pw.w;
+        }
+#pragma unroll
+        for (int q = 0; q < POIS_WIN; ++q) wjm |= (w[q] < T0 ? 1u : 0u) << q;
+        const int32_t inside = N - n0;
+        if (inside < POIS_WIN) wjm &= (1u << inside) - 1u;
+        wfirst = pois_draw_window(j, c.k0, c.k1, plo, phi, n0, w, wjm, *pl);
+      }
+      const uint32_t sh = (uint32_t)(n0 & (POIS_WIN - 1));
+      jm = (wjm >> sh) & 15u;
+      first = wfirst + (uint32_t)__builtin_popcount(wjm & ((1u << sh) - 1u));
     }
     float z[4 * K];
     const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
