@@ -73,10 +73,11 @@ MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2,
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # each scaled by the counter's reading on a 1 GiB device copy (fetch x2.0,
 # write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
-# runs.  Served mix round (k_batch group kernels, 128 requests, seed 0: the
-# partials' writes, which vary with the launch shape, 8.4e7 B with the groups
-# side by side, 1.3e8 B in turn) and the fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = 8.38e7   # HBM bytes of one mix round, copy-calibrated FETCH/WRITE_SIZE
+# runs.  Served mix round (k_batch group kernels, 128 requests, seed 0),
+# keyed by the launch shape's chunks per tile: nearly all of it is the chunk
+# partials' writes, which scatter more with one chunk per tile; and the
+# fixed-increment kernel (2^21 x 500).
+TRAFFIC_MIX_ROUND = {1: 1.327e8, 2: 8.40e7}  # tile_chunks -> HBM bytes of one mix round
 TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 
 
@@ -508,7 +509,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
                      "unit": "Top/s (VALU lane-ops, algorithmic)",
                      "frac": achieved / PEAK_VALU_OPS,
-                     "traffic": TRAFFIC_MIX_ROUND if (args.requests, args.seed) == (128, 0) else None,
+                     "traffic": (TRAFFIC_MIX_ROUND.get(tune.tile_chunks)
+                                 if (args.requests, args.seed) == (128, 0) else None),
                      "kernel": "k_batch (persistent tile kernel)", "kernel_ms": kern_ms},
         "cpu_baseline": cpu,
     }
