@@ -215,7 +215,7 @@ struct KJob {
   // k+1 jumps iff its uniform word w < pois_T[k]; the first mark is the
   // offset of w within its count's interval (see poisson_count)
   int32_t pois_mode;
-  int32_t clock_win;  // arrival-clock mode: steps per jump-queue fill window
+  int32_t clock_win;  // steps per jump-queue fill window (uniform grid), per jump-slot refill (adapted)
   uint32_t pois_T[16];
   double pois_inv[16];  // reciprocal width of count c's interval at [c - 1]
 };
@@ -1554,9 +1554,12 @@ __device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_
   // A live lane takes one step per iteration, so at the start of iteration i
   // every live lane is at row i: the normals of rows i .. i + 3 are drawn for
   // the wavefront together every 4 iterations, in the stream's layout (row r,
-  // normal k is index r K + k), with the jump slots' refill.
+  // normal k is index r K + k).  The second jump slots are refilled every
+  // clock_win iterations (a power of two from 4: about half a jump per path
+  // between refills), a jump beyond them being drawn in the step.
+  const uint32_t refill_mask = (uint32_t)j.clock_win - 1u;
   for (uint32_t row0 = 0; __any(active); row0 += 4) {
-    src.refill(f, active, T32);
+    if ((row0 & refill_mask) == 0u) src.refill(f, active, T32);
     float zq[4 * K];
     src.rows4(row0, active, zq);
 #pragma unroll
@@ -1680,10 +1683,13 @@ __device__ int32_t path_adapted_coupled(const KJob& j, SRC& src, bool lane_valid
   int32_t it = 0;
   overflow = 0;
   int32_t iter = 0;
+  int32_t refill_mask = j.clock_win - 1;  // a power of two minus one
+  while (refill_mask > 0 && (refill_mask + 1) * M > j.clock_win) refill_mask >>= 1;
   while (__any(active)) {
-    // second jump slots for the wavefront together, every 4 iterations (the
-    // jumps keep their Philox blocks, so the marks are those drawn one by one)
-    if ((iter++ & 3) == 0) src.refill(f, active, T32);
+    // second jump slots for the wavefront together, every clock_win / M
+    // iterations (an iteration is M fine steps; the jumps keep their Philox
+    // blocks, so the marks are those drawn one by one)
+    if ((iter++ & refill_mask) == 0) src.refill(f, active, T32);
     if (active && it >= max_steps) {
       overflow = 1;
       active = false;
