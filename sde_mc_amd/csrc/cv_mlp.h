@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_cv_adapted(const KJob job, const uint16
   float bcv = 0.0f, jcv = 0.0f, ccv = 0.0f;
   int32_t iter = 0;
   while (__any(row)) {
-    if ((iter++ & 3) == 0) src.refill(f, active, T32);
+    if ((iter++ & (job.clock_win - 1)) == 0) src.refill(f, active, T32);  // see path_adapted
     if (active && steps >= max_steps) {
       ovf = 1;
       active = false;
