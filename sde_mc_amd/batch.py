@@ -126,6 +126,19 @@ class RoundPlan:
 # a wavefront takes a tile index from its group's counter with one atomic, so
 # short jobs (e.g. MLMC level 0, 4 steps) get several chunks per tile.
 MIN_TILE_COST = 2.0e4
+MAX_TILE_COST = 5.0e4
+
+
+def tile_cost_floor(total_cost, tune, cus):
+    """Least work per tile for a launch of `total_cost` (job_cost units): a
+    large launch can afford bigger tiles (fewer counter atomics and segment
+    searches: the served mix runs 2.5% faster at 5e4 than at 2e4 with one
+    chunk per tile), a small one needs many tiles per resident wavefront to
+    fill the device (the MLMC level set, 3e8 in all, runs 20% slower at
+    5e4); so the floor is the launch's work over 64 tiles per resident
+    wavefront, kept within [MIN_TILE_COST, MAX_TILE_COST]."""
+    waves = max(1, int(cus) * tune.waves_per_block * tune.blocks_per_cu)
+    return float(min(MAX_TILE_COST, max(MIN_TILE_COST, total_cost / (64.0 * waves))))
 
 
 def plan_segments(slots, firsts, counts, costs, tile_chunks, classes=None,
@@ -204,6 +217,7 @@ class BatchRunner:
     def __init__(self, max_jobs=1024, max_segments=None, partial_chunks=1 << 18, tune=None,
                  use_graph=True, n_buffers=1):
         self.device = E.require_device()
+        self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.lib = L.lib()
         self.kb = self.lib.sdemc_sizeof_prepared_job()
         self.max_jobs = int(max_jobs)
@@ -359,7 +373,9 @@ class BatchRunner:
         cls = self.prepare(jobs, base, slots)
         self.upload_slots(range(n))
         costs = np.array([job_cost(j) for j in jobs])
-        plan = plan_segments(slots, np.zeros(n, np.int64), counts, costs, tune.tile_chunks, cls)
+        floor = tile_cost_floor(float(np.dot(counts, costs)), tune, self.cus)
+        plan = plan_segments(slots, np.zeros(n, np.int64), counts, costs, tune.tile_chunks, cls,
+                             min_tile_cost=floor)
         buf = self.bufs[0]
         self.stage_round(buf, plan, slots)
         if events is not None:

@@ -82,6 +82,17 @@ def test_all_classes_fold_into_eight_groups():
     assert sorted(plan.segs[s0:s0 + ns, 0]) == [0, 1, L.CLASS_GENERIC]
 
 
+def test_tile_cost_floor_follows_the_launch():
+    from sde_mc_amd.batch import MAX_TILE_COST, MIN_TILE_COST, BatchTune, tile_cost_floor
+    t = BatchTune(2, 16)  # 32 resident wavefronts per CU
+    # the served mix (1.9e11 cost units) takes the largest tiles, the MLMC
+    # level set (3e8) the smallest; in between, 64 tiles per resident wavefront
+    assert tile_cost_floor(1.9e11, t, 256) == MAX_TILE_COST
+    assert tile_cost_floor(3.0e8, t, 256) == MIN_TILE_COST
+    mid = 64 * 256 * 32 * 3.0e4
+    assert tile_cost_floor(mid, t, 256) == pytest.approx(3.0e4)
+
+
 def test_job_cost_orders_levy_above_merton():
     m = W.request_job(W.Request(0, 'merton', 1.0, 1.0, 200, 1 << 16, 1))
     l = W.request_job(W.Request(1, 'levy', 1.0, 1.0, 200, 1 << 16, 2))
