@@ -77,7 +77,7 @@ MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2,
 # keyed by the launch shape's chunks per tile: nearly all of it is the chunk
 # partials' writes, which scatter more with one chunk per tile; and the
 # fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = {1: 1.327e8, 2: 8.40e7}  # tile_chunks -> HBM bytes of one mix round
+TRAFFIC_MIX_ROUND = {1: 1.134e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
 TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 
 
