@@ -965,18 +965,19 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     jc.init(c.k0, c.k1, plo, phi, f.rate, f.inv_rate);
     kj = step_of(jc.tau, j.n_over_T, N);
   }
-  // Jump draws off the step loop.  The marks of the coming steps are drawn
-  // ahead into a queue of JQ (step, mark) slots, in the order the per-step
-  // form applies them, by passes in which every lane of the wavefront takes
-  // its next jump at once: the draw code then runs about max-over-lanes
-  // times per window instead of once per step in which any lane jumps.  A
-  // step checks only the queue's head.  Arrival-clock mode fills the queue
-  // for each window of clock_win steps (sized by the job's jump rate, rounded
-  // up to whole strides of 4U); per-step Poisson mode for each group of 4
-  // steps, from the group's count words.  Jumps beyond the queue are taken
-  // inside the step loop, by code compiled into the windows (groups) where
-  // some lane needs it.  Marks, order and arithmetic are those of the
-  // per-step form, so results are unchanged.
+  // Jump draws off the step loop.  Arrival-clock mode: the marks of the
+  // coming steps are drawn ahead into a queue of JQ (step, mark) slots, in
+  // the order the per-step form applies them, by passes in which every lane
+  // of the wavefront takes its next jump at once, so the draw code runs
+  // about max-over-lanes times per window of clock_win steps (sized by the
+  // job's jump rate, rounded up to whole strides of 4U) instead of once per
+  // step in which any lane jumps; a step checks only the queue's head, and
+  // jumps beyond the queue are taken inside the step loop by code compiled
+  // into the windows where some lane needs it.  Per-step Poisson mode: the
+  // jumping steps of each window of POIS_WIN steps are drawn by the whole
+  // wavefront from a list in LDS (pois_draw_window), and each step reads its
+  // marks back.  Marks, order and arithmetic are those of the per-step form,
+  // so results are unchanged.
   constexpr int JQ = 2;
   const bool clock = !pois && f.rate > 0.0;  // job-uniform
   PoisLds* const pl = pois ? &pois_lds() : nullptr;
