@@ -172,6 +172,22 @@ def test_rng_paths_match_oracle(model, grid, coupled):
     assert s[2] == P
 
 
+@pytest.mark.parametrize("eps,N,coupled", [(0.05, 101, 0), (0.05, 102, 3), (0.3, 101, 0),
+                                           (0.3, 102, 3)])
+def test_levy_partial_last_jump_window(eps, N, coupled):
+    """Step counts that end inside a jump window (per-step counts draw the
+    jumps of 8 steps at a time, the clock fills its queue per window): the
+    paths still follow the oracle, single level and coupled."""
+    P = 1000
+    job = J.job_for(J.levy_rainbow_sde(eps=eps), 1.0, N, P, coarse_factor=coupled,
+                    option=Rainbow(1.0, log=True), seed=919)
+    assert (oracle.step_jumps(job, 0, 0)[0] < 0) == (eps == 0.3)
+    pay, term, _, s = _run_rng(job)
+    opay, oterm, _ = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+
+
 @pytest.mark.parametrize("eps", [0.05, 0.3])
 def test_levy_stored_paths_in_both_jump_modes(eps):
     """The path-materialising kernel in either uniform-grid jump mode
