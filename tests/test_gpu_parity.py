@@ -172,6 +172,22 @@ def test_rng_paths_match_oracle(model, grid, coupled):
     assert s[2] == P
 
 
+@pytest.mark.parametrize("coupled", [0, 2])
+def test_adapted_low_rate_long_refill_windows(coupled):
+    """lambda h = 0.0015: the adapted grid refills the second jump slots only
+    every 128 iterations (64 coarse iterations of a coupled pair), so lanes
+    with two jumps inside one window draw the next in the step; the paths and
+    step counts still follow the oracle."""
+    P, N = 1500, 1000
+    job = J.job_for(J.merton_sde(rate=1.5), 1.0, N, P, grid='adapted', coarse_factor=coupled,
+                    option=EuroCall(1.0), disc=math.exp(-0.02), seed=929)
+    pay, term, ns, s = _run_rng(job)
+    opay, oterm, ons = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    assert np.array_equal(ns, ons)
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+
+
 @pytest.mark.parametrize("eps,N,coupled", [(0.05, 101, 0), (0.05, 102, 3), (0.3, 101, 0),
                                            (0.3, 102, 3)])
 def test_levy_partial_last_jump_window(eps, N, coupled):
