@@ -63,13 +63,20 @@ def test_job_alone_equals_job_in_mix():
 
 def test_graph_replays_are_consistent():
     jobs = _mix(8, seed=5)
+    other = _mix(6, seed=9)
     r = BatchRunner(max_jobs=64, use_graph=True)
+    r2 = BatchRunner(max_jobs=64, use_graph=False)
+    other_eager = r2.run(other)
     first = r.run(jobs)
-    for _ in range(4):
+    for i in range(4):
         torch.cuda.synchronize()
         again = r.run(jobs)
-        assert np.array_equal(again, first)
-    r2 = BatchRunner(max_jobs=64, use_graph=False)
+        assert np.array_equal(again, first), i
+        # eager device work between replays: a plain launch, a torch
+        # reduction, and another job set through the same captured plans
+        assert np.array_equal(_plain(other[i]), other_eager[i])
+        torch.arange(1 << 20, device=r.device, dtype=torch.float64).sum().item()
+        assert np.array_equal(r.run(other), other_eager), i
     assert np.array_equal(r2.run(jobs), first)
     r3 = BatchRunner(max_jobs=64, use_graph=False, tune=BatchTune(concurrent_groups=0))
     assert np.array_equal(r3.run(jobs), first)
