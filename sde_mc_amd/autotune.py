@@ -98,10 +98,10 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confi
     With `confirm_jobs` (the real batch, when `jobs` is a cut-down copy of
     it), the `confirm_top` fastest shapes of the sweep are timed again on
     `confirm_jobs`, each also with its class groups run one after another
-    instead of side by side, and the fastest there is kept (or the fastest
-    in-turn shape, if it is within CONCURRENT_MARGIN of a side-by-side
-    winner): a copy with capped path counts ranks close shapes differently
-    from the full-size batch."""
+    instead of side by side, and the fastest there is kept: a copy with
+    capped path counts ranks close shapes differently from the full-size
+    batch.  Either way, the fastest in-turn shape replaces a side-by-side
+    winner that beats it by no more than CONCURRENT_MARGIN."""
     classes = sorted(set(job_class(j) for j in jobs))
     key = '%s/classes%s' % (_device_key(), '-'.join(str(c) for c in classes))
     cache = load_cache() if use_cache else {}
@@ -117,7 +117,7 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confi
         if verbose:
             print('autotune %r: %.3f ms' % (tune, ms), flush=True)
     runner.close()
-    best = min(table, key=lambda r: r[1])
+    timed = table
     if confirm_jobs is not None:
         ranked = [k for k, _ in sorted(table, key=lambda r: r[1])[:confirm_top]]
         # and each of them with the class groups one after another: several
@@ -131,12 +131,13 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confi
         if verbose:
             for k, ms in confirmed:
                 print('autotune confirm %r: %.3f ms' % (k, ms), flush=True)
-        best = min(confirmed, key=lambda r: r[1])
-        in_turn = [r for r in confirmed if r[0][3] == 0]
-        if best[0][3] != 0 and in_turn:
-            bt = min(in_turn, key=lambda r: r[1])
-            if bt[1] <= best[1] * (1.0 + CONCURRENT_MARGIN):
-                best = bt
+        timed = confirmed
+    best = min(timed, key=lambda r: r[1])
+    in_turn = [r for r in timed if r[0][3] == 0]
+    if best[0][3] != 0 and in_turn:
+        bt = min(in_turn, key=lambda r: r[1])
+        if bt[1] <= best[1] * (1.0 + CONCURRENT_MARGIN):
+            best = bt
     cache = load_cache()
     cache[key] = {'best': list(best[0]), 'table': table, 'seconds': time.time() - t0}
     if use_cache:
