@@ -165,3 +165,48 @@ def test_job_classes():
     bad.steps = 0
     with pytest.raises(L.SdemcError):
         job_class(bad)
+
+
+def test_autotune_prefers_in_turn_groups_within_the_margin(monkeypatch):
+    from types import SimpleNamespace
+    from sde_mc_amd import autotune as A
+    from sde_mc_amd.batch import BatchTune
+
+    class NoRunner:
+        def __init__(self, **kw):
+            pass
+
+        def close(self):
+            pass
+
+    side, turn, other = BatchTune(4, 4, 2, 1, 1), BatchTune(2, 16, 1, 0, 1), BatchTune(8, 2, 1, 1, 1)
+    sweep, confirm = {}, {}
+    monkeypatch.setattr(A, 'BatchRunner', NoRunner)
+    monkeypatch.setattr(A, 'job_class', lambda j: 0)
+    monkeypatch.setattr(A, '_device_key', lambda: 'host')
+    monkeypatch.setattr(A, 'time_tune', lambda r, jobs, t, reps=2: sweep[t.key()])
+    monkeypatch.setattr(A, 'time_tunes_interleaved',
+                        lambda r, jobs, keys, rounds=3: [(k, confirm[k]) for k in keys])
+    jobs = [SimpleNamespace(n_paths=1 << 12)]
+    cands = [side, turn, other]
+
+    def pick(times, confirm_times=None):
+        sweep.clear()
+        sweep.update({t.key(): ms for t, ms in zip(cands, times)})
+        kw = {}
+        if confirm_times is not None:
+            confirm.clear()
+            for t, ms in zip(cands, confirm_times):
+                confirm[t.key()] = ms
+                flip = t.key()[:3] + (1 - t.key()[3],) + t.key()[4:]
+                confirm.setdefault(flip, 99.0)
+            kw = dict(confirm_jobs=jobs, confirm_top=3)
+        return A.autotune(jobs, candidates=cands, use_cache=False, **kw)[0].key()
+
+    assert A.CONCURRENT_MARGIN == 0.02
+    assert pick([10.0, 10.15, 11.0]) == turn.key()   # 1.5% behind: in turn kept
+    assert pick([10.0, 10.3, 11.0]) == side.key()    # 3% behind: side by side kept
+    assert pick([11.0, 12.0, 10.0]) == other.key()   # no in-turn shape close
+    # the confirm stage's times decide, not the sweep's
+    assert pick([10.0, 10.3, 11.0], [10.0, 10.1, 12.0]) == turn.key()
+    assert pick([10.0, 10.1, 11.0], [10.0, 10.5, 12.0]) == side.key()
