@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDEMC_ABI_VERSION 3
+#define SDEMC_ABI_VERSION 4
 #define SDEMC_MAX_DIM 4
 #define SDEMC_CHUNK 64 /* paths per reduction chunk (one wavefront) */
 
@@ -192,10 +192,19 @@ typedef struct sdemc_fixed_in {
 
 /* ---- statistics ------------------------------------------------------------
  * stats_dev receives 4 doubles per job: {sum, sum of squares, n, flags}
- * where flags != 0 marks a path that overflowed max_steps.  The sums are taken
- * in fp64 over fixed 64-path chunks and then over chunks in a fixed order, so
- * a job's result depends only on the job, never on the launch it ran in.
+ * where flags is an OR of enum sdemc_flag bits (0: a valid result).  The sums
+ * are taken in fp64 over fixed 64-path chunks and then over chunks in a fixed
+ * order, so a job's result depends only on the job, never on the launch it
+ * ran in.
  */
+enum sdemc_flag {
+  SDEMC_FLAG_OVERFLOW = 1, /* a path needed more grid steps than max_steps          */
+  SDEMC_FLAG_JUMP_CSR = 2, /* fixed mode: a jump mark read outside jump_size         */
+  SDEMC_FLAG_CLASS = 4,    /* batched: a job outside its group's kernel class        */
+  SDEMC_FLAG_TABLE = 8     /* batched: the launch's segment table does not describe
+                              the prepared job table (see sdemc_batch_check); the
+                              sums are not a result                                 */
+};
 
 /* Library identity. */
 int sdemc_abi_version(void);
@@ -312,8 +321,11 @@ int sdemc_cv_simulate(const sdemc_job* job, const sdemc_cv_spec* cv, double* sta
  * tiles are handed out.  Persistent wavefronts take tile indices from a
  * device counter until all tiles are taken, so jobs of any size and any mix
  * of models, grids and MLMC levels share one launch.  Partials of slot j
- * live at partials[chunk_base_j + c].  Segments naming an unprepared slot or
- * chunks outside a slot are skipped.
+ * live at partials[chunk_base_j + c].  A launch whose segments name an
+ * unprepared slot, chunks outside a slot or the partial pool, or tile ranges
+ * that do not cover their chunks runs what it can and sets SDEMC_FLAG_TABLE in
+ * every slot it finalises; sdemc_batch_check finds the same faults on the
+ * host before a launch.
  */
 typedef struct sdemc_segment {
   int32_t job;        /* slot index                                     */
@@ -368,8 +380,9 @@ typedef struct sdemc_batch_dev {
   double* partials_dev;          /* [partial_chunks][2]                         */
   int32_t* chunk_flags_dev;      /* [partial_chunks] flags                      */
   double* stats_dev;             /* [max_jobs][4]                               */
-  int32_t* counter_dev;          /* [SDEMC_MAX_GROUPS] tile counters, zeroed by
-                                    the launch                                  */
+  int32_t* counter_dev;          /* [SDEMC_MAX_GROUPS + 1]: the groups' tile
+                                    counters, then the launch's fault word; all
+                                    zeroed by the launch                        */
   int64_t partial_chunks;        /* capacity of partials / chunk_flags          */
   int32_t max_jobs;
   int32_t max_segments;
@@ -388,6 +401,20 @@ typedef struct sdemc_batch_tune {
   int32_t step_unroll;       /* 4-step groups per loop iteration of the uniform
                                 single-level path: 0 or 1 (one), 2 (two)       */
 } sdemc_batch_tune;
+
+/* Host-side check of a launch's tables before they are uploaded: the
+ * prepared slots (host copy, max_jobs * sdemc_sizeof_prepared_job() bytes),
+ * the segment list, the header [SDEMC_MAX_GROUPS][4] and the group classes.
+ * Per group: the header names segments inside the list; their tile ranges
+ * run consecutively from 0, each of ceil(count / per_tile) tiles, and add up
+ * to the header's tile count; each segment names a prepared slot of the
+ * group's class (any class in the generic group), chunks inside that slot's
+ * job and partial slots inside the pool.  SDEMC_OK or SDEMC_ERR_ARG.
+ * No counterpart in the reference (it has no batched launch); it replaces
+ * the planner's trust in its own tables with a check. */
+int sdemc_batch_check(const void* prepared_host, int32_t max_jobs, const sdemc_segment* segs,
+                      int32_t max_segments, const int32_t* hdr, int32_t n_groups,
+                      const int32_t* group_class, int64_t partial_chunks);
 
 /* One batched launch: zero the tile counters, run the group kernels,
  * finalise the flagged slots.  Everything is ordered after earlier work on

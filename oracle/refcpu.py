@@ -1,5 +1,5 @@
 """TEST INFRASTRUCTURE ONLY: torch-CPU restatement of the reference's solver
-loop, op for op, for affine models (Merton, GBM, the Levy models).
+loops, op for op, for affine models (Merton, GBM, the Levy models).
 
 This is the "reference CPU path" that bench.py times on the host cores and
 that tests use to cross-check the C oracle's event grid: the same fp32
@@ -7,7 +7,9 @@ tensor operations per step as JumpDiffusionSolver.solve
 (sde_mc/solvers.py:164-226) -- per-step host loop, dense path storage,
 a jump mark drawn for every path at every step (solvers.py:211) -- with the
 model's coefficients written as affine functions (sde.py:368-375,
-levy.py:74-80, 113-120) and the payoff of mc.py:87-100.
+levy.py:74-80, 113-120) and the payoff of mc.py:87-100; and the coupled
+fine/coarse loop of JumpDiffusionSolver.multilevel_solve
+(solvers.py:228-307) on fixed draws (multilevel_adapted).
 
 Randomness comes either from a torch.Generator (timing) or from fixed host
 arrays (cross-checks): normals [S][K][P] standard, jump times and marks
@@ -70,6 +72,19 @@ class TruncatedPowerIcdf:
             x1 = x4 = torch.zeros_like(y)
         return torch.where(y <= self.y1, x1, torch.where(y < self.y2, x2,
                                                          torch.where(y < self.y3, x3, x4)))
+
+
+def model_from_kernel(km):
+    """AffineModel carrying the coefficients of a kernel model description
+    (an sdemc_model: fields dim, noise, rate, x0, a0, a1, b0, b1, c0, c1, j0,
+    j1, chol row-major with stride 4).  Noise 1 (the jump solver's 'indep':
+    one common second driver, solvers.py:198-201) maps to 'common'."""
+    d = int(km.dim)
+    v = lambda f: [float(getattr(km, f)[i]) for i in range(d)]
+    chol = [[float(km.chol[4 * i + j]) for j in range(d)] for i in range(d)]
+    return AffineModel(v('x0'), v('a0'), v('a1'), v('b0'), v('b1'), v('c0'), v('c1'), v('j0'),
+                       v('j1'), chol=chol, noise='common' if int(km.noise) == 1 else 'diag',
+                       rate=float(km.rate))
 
 
 def merton_model(mu, sigma, lam, alpha, gamma, x0=1.0):
@@ -141,3 +156,91 @@ def price(model, payoff, disc, T, N, n, max_jumps, gen):
     paths, total = solve_adapted(model, T, N, n, max_jumps, gen=gen)
     pay = payoff(paths[:, total]) * disc
     return float(pay.mean()), float(pay.std() / math.sqrt(n))
+
+
+def _affine_step(model, x, dt, corr, common):
+    """EulerScheme.step (schemes.py:5-10) for the affine coefficients:
+    x + a(x) dt + b(x) dW (+ c(x) dB, the common second driver)."""
+    diff = (model.b0 + model.b1 * x) * corr
+    if common is not None:
+        diff = diff + (model.c0 + model.c1 * x) * common
+    return x + (model.a0 + model.a1 * x) * dt + diff
+
+
+def multilevel_adapted(model, T, fine, coarse, bs, fixed, exact_jumps=False, plant=None):
+    """JumpDiffusionSolver.multilevel_solve (solvers.py:228-307) op for op in
+    torch fp32, on fixed draws.
+
+    fixed = (z [S][K][bs] standard normals, jump_times [J][bs], marks [J][bs]):
+    fine sub-step number k (counted from 0 over the whole loop, the
+    reference's total_steps_fine - 1) takes row k of z; K = dim, plus one
+    common column for noise 'common'.  Returns (x_fine [bs, d], x_coarse
+    [bs, d], iterations [bs]): the terminal states and, per path, the loop
+    iterations (one coarse step each) it took while its fine clock was below
+    T -- the reference keeps stepping finished paths with zero-length fine
+    steps until every path is done, which leaves them unchanged.
+
+    plant: None, or the name of a deliberate mistake for tests that show a
+    comparison with this loop can fail ('coarse_post_state': the coarse jump
+    coefficient at the post-step state; 'coarse_untruncated': the coarse
+    step not cut at the jump time; 'sum_unscaled': the coarse increment sums
+    the unscaled normals of the fine sub-steps)."""
+    d = model.dim
+    factor = int(fine / coarse)
+    h_fine = torch.tensor(T / fine)
+    h_coarse = factor * h_fine
+    t_fine = torch.zeros((bs, 1))
+    t_coarse = torch.zeros((bs, 1))
+    x_fine = model.x0.unsqueeze(0).repeat(bs, 1)
+    x_coarse = model.x0.unsqueeze(0).repeat(bs, 1)
+    zfix = torch.as_tensor(fixed[0])
+    jump_times = torch.as_tensor(np.ascontiguousarray(fixed[1].T)).unsqueeze(-1)
+    marks = torch.as_tensor(np.ascontiguousarray(fixed[2].T))
+    jump_idx = torch.zeros((bs, 1), dtype=torch.long)
+    iters = torch.zeros(bs, dtype=torch.long)
+    zero = torch.tensor(0.)
+    total_fine = 0
+    while torch.any(t_fine < T):
+        iters += (t_fine.squeeze(-1) < T).long()
+        run_sum = 0
+        run_sum_common = 0
+        nxt = jump_times[torch.arange(bs), jump_idx.clamp(max=jump_times.shape[1] - 1).squeeze(-1), :]
+        for _ in range(factor):
+            total_fine += 1
+            h_fine = torch.minimum(h_fine, torch.maximum(T - t_fine, zero))
+            dt_fine = torch.minimum(h_fine, nxt - t_fine)
+            sq = torch.sqrt(dt_fine.unsqueeze(-1))
+            row = zfix[total_fine - 1].T  # (bs, K)
+            z = row[:, :d].unsqueeze(-1)
+            corr = torch.matmul(model.L, z * sq).squeeze(-1)
+            common = None
+            if model.noise == 'common':
+                common = (row[:, d:d + 1].unsqueeze(-1) * sq).squeeze(-1).repeat(1, d)
+            old_x_fine = x_fine
+            x_fine = _affine_step(model, x_fine, dt_fine, corr, common)
+            t_fine = t_fine + dt_fine
+            if plant == 'sum_unscaled':
+                run_sum = run_sum + torch.matmul(model.L, z).squeeze(-1)
+            else:
+                run_sum = run_sum + corr
+            if common is not None:
+                run_sum_common = run_sum_common + common
+        h_coarse = torch.minimum(h_coarse, torch.maximum(T - t_coarse, zero))
+        if plant == 'coarse_untruncated':
+            dt_coarse = h_coarse
+        else:
+            dt_coarse = torch.minimum(h_coarse, nxt - t_coarse)
+        old_x_coarse = x_coarse
+        x_coarse = _affine_step(model, x_coarse, dt_coarse, run_sum,
+                                run_sum_common if model.noise == 'common' else None)
+        t_coarse = t_coarse + dt_coarse
+        hit = torch.isclose(nxt, t_fine, atol=1e-12)
+        idx = jump_idx.squeeze(-1).clamp(max=marks.shape[1] - 1)
+        mark = marks[torch.arange(bs), idx].unsqueeze(-1).repeat(1, d)
+        cur = torch.where(hit, mark, torch.zeros_like(mark))
+        yf = x_fine if exact_jumps else old_x_fine
+        yc = x_coarse if (exact_jumps or plant == 'coarse_post_state') else old_x_coarse
+        x_fine = x_fine + (model.j0 + model.j1 * yf) * cur
+        x_coarse = x_coarse + (model.j0 + model.j1 * yc) * cur
+        jump_idx = torch.where(hit, jump_idx + 1, jump_idx)
+    return x_fine, x_coarse, iters

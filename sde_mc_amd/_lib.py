@@ -14,7 +14,9 @@ CHUNK = 64
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_WORKSPACE, ERR_OVERFLOW, ERR_DEVICE = range(7)
 
-ABI_VERSION = 3   # SDEMC_ABI_VERSION these structures follow
+ABI_VERSION = 4   # SDEMC_ABI_VERSION these structures follow
+# stats flag bits (enum sdemc_flag)
+FLAG_OVERFLOW, FLAG_JUMP_CSR, FLAG_CLASS, FLAG_TABLE = 1, 2, 4, 8
 
 NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON = 0, 1, 2, 3
 JUMP_LOGNORMAL, JUMP_LEVY = 0, 1
@@ -142,6 +144,9 @@ _SIGS = {
     "sdemc_prepare_jobs": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int32)]),
+    "sdemc_batch_check": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_int64]),
     "sdemc_batch_launch": (ctypes.c_int, [ctypes.POINTER(BatchDev), ctypes.POINTER(BatchTune),
                                           ctypes.c_void_p]),
     "sdemc_batch_plan_create": (ctypes.c_int, [ctypes.POINTER(BatchDev),

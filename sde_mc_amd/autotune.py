@@ -6,6 +6,7 @@ unroll -- on a calibration batch, timing the tile kernel with device events, and
 fastest shape per (device, set of kernel classes in the batch).  Results are cached in a JSON
 file next to the library so later processes start tuned.
 """
+import hashlib
 import json
 import os
 import time
@@ -41,6 +42,19 @@ def cache_path():
 def _device_key():
     p = torch.cuda.get_device_properties(torch.cuda.current_device())
     return '%s/%d' % (getattr(p, 'gcnArchName', p.name), p.multi_processor_count)
+
+
+def _build_key(candidates):
+    """What a cached pick depends on besides the device and the classes: the
+    library's ABI version, the built library itself (size and modification
+    time, so a rebuild of the kernels retunes) and the candidate list."""
+    try:
+        st = os.stat(L.lib_path())
+        stamp = '%d-%d' % (st.st_size, int(st.st_mtime))
+    except OSError:
+        stamp = 'nolib'
+    cands = hashlib.sha1(repr([c.key() for c in candidates]).encode()).hexdigest()[:12]
+    return 'abi%d/lib%s/cand%s' % (L.ABI_VERSION, stamp, cands)
 
 
 def load_cache():
@@ -103,7 +117,8 @@ def autotune(jobs, candidates=None, reps=2, use_cache=True, verbose=False, confi
     batch.  Either way, the fastest in-turn shape replaces a side-by-side
     winner that beats it by no more than CONCURRENT_MARGIN."""
     classes = sorted(set(job_class(j) for j in jobs))
-    key = '%s/classes%s' % (_device_key(), '-'.join(str(c) for c in classes))
+    key = '%s/classes%s/%s' % (_device_key(), '-'.join(str(c) for c in classes),
+                                _build_key(candidates or CANDIDATES))
     cache = load_cache() if use_cache else {}
     if use_cache and key in cache:
         return BatchTune(*cache[key]['best']), cache[key]['table']

@@ -228,7 +228,8 @@ class BatchRunner:
         self.jobs = torch.zeros(self.max_jobs * self.kb, dtype=torch.uint8, device=dev)
         self.h_jobs = torch.zeros(self.max_jobs * self.kb, dtype=torch.uint8, pin_memory=True)
         self.stats = torch.zeros(self.max_jobs * 4, dtype=torch.float64, device=dev)
-        self.counter = torch.zeros(L.MAX_GROUPS, dtype=torch.int32, device=dev)
+        # the groups' tile counters, then the launch's fault word
+        self.counter = torch.zeros(L.MAX_GROUPS + 1, dtype=torch.int32, device=dev)
         self.slot_class = np.full(self.max_jobs, L.CLASS_GENERIC, dtype=np.int32)
         self.bufs = [RoundBuffer(dev, self.max_jobs, self.max_segments) for _ in range(n_buffers)]
         self._plans = {}
@@ -306,6 +307,16 @@ class BatchRunner:
             self.jobs[a:b].copy_(self.h_jobs[a:b], non_blocking=True)
             i = j + 1
 
+    def check_tables(self, buf, groups):
+        """sdemc_batch_check of the staged round against the host copy of the
+        job table: raises SdemcError before a launch could run a mis-planned
+        segment list."""
+        groups = tuple(int(g) for g in groups) or (L.CLASS_GENERIC,)
+        gc = (ctypes.c_int32 * len(groups))(*groups)
+        L.check(self.lib.sdemc_batch_check(
+            self.h_jobs.data_ptr(), self.max_jobs, buf.h_segs.data_ptr(), buf.max_segments,
+            buf.h_hdr.data_ptr(), len(groups), gc, self.partial_chunks), 'sdemc_batch_check')
+
     def stage_round(self, buf, plan, finalize_slots):
         segs = plan.segs
         n = len(segs)
@@ -313,8 +324,10 @@ class BatchRunner:
             raise ValueError('%d segments exceed max_segments %d' % (n, buf.max_segments))
         if n:
             buf.h_segs.numpy()[:NS * n] = segs.reshape(-1)
-            buf.segs[:NS * n].copy_(buf.h_segs[:NS * n], non_blocking=True)
         buf.h_hdr.numpy()[:] = plan.hdr.reshape(-1)
+        self.check_tables(buf, plan.groups)
+        if n:
+            buf.segs[:NS * n].copy_(buf.h_segs[:NS * n], non_blocking=True)
         buf.hdr.copy_(buf.h_hdr, non_blocking=True)
         hf = buf.h_final.numpy()
         hf[:] = 0

@@ -65,6 +65,10 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr)
     os.replace(tmp, out)
+    # launch shapes tuned on the previous build no longer apply
+    tuned = os.path.join(LIBDIR, "autotune.json")
+    if os.path.exists(tuned):
+        os.remove(tuned)
     if verbose:
         print("built", out)
     return out

@@ -69,19 +69,28 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
 // carries the other mode's registers: 1-D arrival clock (class 0), 2-D
 // per-step Poisson counts (class 4) and 2-D arrival clock (class 8); 1-D
 // per-step count jobs run in the generic kernel.
-inline int32_t job_class_of(const sdemc_job& j) {
-  const int noise = j.model.noise, dim = j.model.dim, grid = j.grid;
-  const bool coupled = j.coarse_factor > 0;
+inline int32_t class_of(int noise, int dim, int grid, bool coupled, bool pois) {
   int32_t fam = -1;
   if (noise == SDEMC_NOISE_DIAG && dim == 1) fam = 0;
   if (noise == SDEMC_NOISE_INDEP_COMMON && dim == 2) fam = 1;
   if (fam < 0) return SDEMC_CLASS_GENERIC;
   if (grid == SDEMC_GRID_UNIFORM && !coupled) {
-    const bool pois = job_pois_mode(j);
     if (fam == 0) return pois ? SDEMC_CLASS_GENERIC : SDEMC_CLASS_DIAG1;
     return pois ? SDEMC_CLASS_INDEP2 : SDEMC_CLASS_INDEP2_CLOCK;
   }
   return fam * 4 + grid * 2 + (coupled ? 1 : 0);
+}
+
+inline int32_t job_class_of(const sdemc_job& j) {
+  return class_of(j.model.noise, j.model.dim, j.grid, j.coarse_factor > 0, job_pois_mode(j));
+}
+
+// The class of a prepared slot, decoded from its dispatch key
+// ((noise * 4 + dim - 1) * 2 + grid) * 2 + coupled and its jump mode.
+inline int32_t kjob_class_of(const KJob& k) {
+  const int key = k.class_key;
+  return class_of(key >> 4, ((key >> 2) & 3) + 1, (key >> 1) & 1, (key & 1) != 0,
+                  k.pois_mode != 0);
 }
 
 #define SDEMC_CASE(NOISE, D, FAM, GRID, C)                              \
@@ -145,6 +154,16 @@ __device__ __forceinline__ bool run_chunk(const KJob& job, int64_t chunk, double
 #undef SDEMC_CASE
 #undef SDEMC_CASES_DG
 
+// A launch whose segment table does not describe the prepared job table
+// (a segment naming an unprepared slot, chunks outside its job or the
+// partial pool, tiles outside their segment, a header outside the segment
+// list) computes what it can and sets the launch's fault word; every slot
+// finalised by that launch then carries flag SDEMC_FLAG_TABLE, so a
+// mis-planned round never returns stale statistics as a result.
+__device__ __forceinline__ void raise_fault(int32_t* fault, int lane) {
+  if (lane == 0) *fault = 1;
+}
+
 // Persistent tile kernel of one class group.  hdr = {first segment,
 // segments, tiles}; tile indices come from this group's counter.  U is the
 // step unroll of the uniform single-level path (classes 0 and 4).
@@ -155,14 +174,22 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
                                                double* __restrict__ partials,
                                                int32_t* __restrict__ cflags,
                                                int32_t* __restrict__ counter,
+                                               int32_t* __restrict__ fault,
                                                int64_t partial_chunks, int32_t max_jobs,
                                                int32_t max_segments) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int32_t s0 = hdr[0];
   int32_t nseg = hdr[1];
   const int32_t ntiles = hdr[2];
-  if (s0 < 0 || s0 >= max_segments || nseg <= 0) return;
-  nseg = nseg < max_segments - s0 ? nseg : max_segments - s0;
+  if (ntiles <= 0) return;  // nothing to run in this group
+  if (s0 < 0 || s0 >= max_segments || nseg <= 0) {
+    raise_fault(fault, lane);
+    return;
+  }
+  if (nseg > max_segments - s0) {
+    raise_fault(fault, lane);
+    nseg = max_segments - s0;
+  }
   int32_t seg = s0;
   const int32_t last = s0 + nseg - 1;
   for (;;) {
@@ -181,47 +208,76 @@ __global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
     }
     seg = lo;
     const sdemc_segment sg = segs[seg];
-    if (sg.job < 0 || sg.job >= max_jobs || sg.per_tile < 1) continue;
+    if (sg.job < 0 || sg.job >= max_jobs || sg.per_tile < 1 || sg.count < 1 || sg.first < 0 ||
+        t < sg.tile_begin) {
+      raise_fault(fault, lane);
+      continue;
+    }
     const KJob& job = jobs[sg.job];
-    if (job.magic != KJOB_MAGIC) continue;
+    if (job.magic != KJOB_MAGIC) {
+      raise_fault(fault, lane);
+      continue;
+    }
     const int64_t nc = (int64_t)((job.n_paths + WAVE - 1) / WAVE);
-    const int64_t seg_end = (int64_t)sg.first + (sg.count > 0 ? sg.count : 0);
-    int64_t c0 = (int64_t)sg.first + (int64_t)(t - sg.tile_begin) * sg.per_tile;
+    const int64_t seg_end = (int64_t)sg.first + sg.count;
+    const int64_t c0 = (int64_t)sg.first + (int64_t)(t - sg.tile_begin) * sg.per_tile;
     int64_t c1 = c0 + sg.per_tile;
+    // the segment's last tile (the one before the next segment's first)
+    // must reach its last chunk, or chunks of it would run in no tile
+    const int32_t next_begin = seg < last ? segs[seg + 1].tile_begin : ntiles;
+    const bool short_range = t == next_begin - 1 && c1 < seg_end;
     if (c1 > seg_end) c1 = seg_end;
-    if (c1 > nc) c1 = nc;
-    if (c0 < 0) c0 = 0;
+    // a tile past its segment's chunks, or chunks past the job's
+    if (short_range || c0 >= seg_end || seg_end > nc || job.chunk_base < 0 ||
+        job.chunk_base + seg_end > partial_chunks) {
+      raise_fault(fault, lane);
+      if (c0 >= seg_end || seg_end > nc || job.chunk_base < 0 ||
+          job.chunk_base + seg_end > partial_chunks)
+        continue;
+    }
     for (int64_t chunk = c0; chunk < c1; ++chunk) {
       const int64_t slot = job.chunk_base + chunk;
-      if (slot < 0 || slot >= partial_chunks) break;
       if (!run_chunk<CLS, U>(job, chunk, partials + 2 * slot, cflags + slot) && lane == 0) {
         // a job outside this kernel's class: poison its partial and flag it
         partials[2 * slot] = __builtin_nan("");
         partials[2 * slot + 1] = __builtin_nan("");
-        cflags[slot] = 4;
+        cflags[slot] = SDEMC_FLAG_CLASS;
       }
     }
   }
 }
 
+// Zero the launch's n tile counters and its fault word (counter[SDEMC_MAX_GROUPS]).
 __global__ void k_zero_counter(int32_t* counter, int32_t n) {
   if ((int32_t)threadIdx.x < n) counter[threadIdx.x] = 0;
+  if (threadIdx.x == 0) counter[SDEMC_MAX_GROUPS] = 0;
 }
 
-// Fixed-order finalisation of the slots flagged this launch.
-
+// Fixed-order finalisation of the slots flagged this launch.  A slot that is
+// not prepared or whose chunks lie outside the partial pool gets NaN sums and
+// flag SDEMC_FLAG_TABLE; so does every slot when the launch's fault word is set.
 __global__ __launch_bounds__(256) void k_finalize_slots(const KJob* __restrict__ jobs,
                                                         const int32_t* __restrict__ mask,
                                                         const double* partials,
                                                         const int32_t* cflags,
+                                                        const int32_t* __restrict__ fault,
                                                         int64_t partial_chunks, double* stats) {
   const int j = blockIdx.x;
   if (mask[j] == 0) return;
   const KJob& job = jobs[j];
-  if (job.magic != KJOB_MAGIC) return;
+  double* st = stats + 4 * (int64_t)j;
   const int64_t nc = (int64_t)((job.n_paths + WAVE - 1) / WAVE);
-  if (job.chunk_base < 0 || job.chunk_base + nc > partial_chunks) return;
-  finalize_block(partials, cflags, job.chunk_base, nc, job.n_paths, stats + 4 * (int64_t)j);
+  if (job.magic != KJOB_MAGIC || job.chunk_base < 0 || job.chunk_base + nc > partial_chunks) {
+    if (threadIdx.x == 0) {
+      st[0] = __builtin_nan("");
+      st[1] = __builtin_nan("");
+      st[2] = 0.0;
+      st[3] = (double)SDEMC_FLAG_TABLE;
+    }
+    return;
+  }
+  finalize_block(partials, cflags, job.chunk_base, nc, job.n_paths, st,
+                 *fault ? SDEMC_FLAG_TABLE : 0);
 }
 
 static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {
@@ -257,7 +313,8 @@ static void launch_group(const sdemc_batch_dev& b, int g, unsigned grid, int wpb
   hipLaunchKernelGGL((k_batch<CLS, U>), dim3(grid), dim3(WAVE * wpb),
                      (size_t)wpb * sizeof(PoisLds), s, (const KJob*)b.jobs_dev,
                      b.segs_dev, b.seg_hdr_dev + 4 * g, b.partials_dev, b.chunk_flags_dev,
-                     b.counter_dev + g, b.partial_chunks, b.max_jobs, b.max_segments);
+                     b.counter_dev + g, b.counter_dev + SDEMC_MAX_GROUPS, b.partial_chunks,
+                     b.max_jobs, b.max_segments);
 }
 
 // Streams and events that fork a round's group kernels off the caller's
@@ -357,7 +414,7 @@ static int enqueue_batch(const sdemc_batch_dev& b, const sdemc_batch_tune& tune,
   }
   hipLaunchKernelGGL(k_finalize_slots, dim3((unsigned)b.max_jobs), dim3(256), 0, s,
                      (const KJob*)b.jobs_dev, b.finalize_dev, b.partials_dev, b.chunk_flags_dev,
-                     b.partial_chunks, b.stats_dev);
+                     b.counter_dev + SDEMC_MAX_GROUPS, b.partial_chunks, b.stats_dev);
   return SDEMC_OK;
 }
 
@@ -393,6 +450,38 @@ int sdemc_prepare_jobs(const sdemc_job* jobs, int32_t n, const int64_t* chunk_ba
     memcpy((char*)prepared_out + (size_t)i * sizeof(KJob), &k, sizeof(KJob));
     if (classes)
       classes[i] = job_class_of(jobs[i]);
+  }
+  return SDEMC_OK;
+}
+
+int sdemc_batch_check(const void* prepared_host, int32_t max_jobs, const sdemc_segment* segs,
+                      int32_t max_segments, const int32_t* hdr, int32_t n_groups,
+                      const int32_t* group_class, int64_t partial_chunks) {
+  if (!prepared_host || !segs || !hdr || !group_class) return SDEMC_ERR_ARG;
+  if (max_jobs < 1 || max_segments < 1 || partial_chunks < 0) return SDEMC_ERR_ARG;
+  if (n_groups < 1 || n_groups > SDEMC_MAX_GROUPS) return SDEMC_ERR_ARG;
+  const KJob* jobs = (const KJob*)prepared_host;
+  for (int g = 0; g < n_groups; ++g) {
+    const int32_t cls = group_class[g];
+    if (cls < 0 || cls > SDEMC_CLASS_GENERIC) return SDEMC_ERR_ARG;
+    const int32_t first = hdr[4 * g], n = hdr[4 * g + 1], tiles = hdr[4 * g + 2];
+    if (n == 0 && tiles == 0) continue;
+    if (first < 0 || n < 1 || first > max_segments - n || tiles < 1) return SDEMC_ERR_ARG;
+    int64_t next_tile = 0;
+    for (int32_t i = first; i < first + n; ++i) {
+      const sdemc_segment& sg = segs[i];
+      if (sg.tile_begin != next_tile) return SDEMC_ERR_ARG;
+      if (sg.job < 0 || sg.job >= max_jobs || sg.per_tile < 1 || sg.count < 1 || sg.first < 0)
+        return SDEMC_ERR_ARG;
+      const KJob& k = jobs[sg.job];
+      if (k.magic != KJOB_MAGIC) return SDEMC_ERR_ARG;
+      if (cls != SDEMC_CLASS_GENERIC && kjob_class_of(k) != cls) return SDEMC_ERR_ARG;
+      const int64_t nc = (int64_t)((k.n_paths + WAVE - 1) / WAVE);
+      const int64_t end = (int64_t)sg.first + sg.count;
+      if (end > nc || k.chunk_base < 0 || k.chunk_base + end > partial_chunks) return SDEMC_ERR_ARG;
+      next_tile += (sg.count + sg.per_tile - 1) / sg.per_tile;
+    }
+    if (next_tile != tiles) return SDEMC_ERR_ARG;
   }
   return SDEMC_OK;
 }

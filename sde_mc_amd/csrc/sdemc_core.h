@@ -818,7 +818,9 @@ __device__ __forceinline__ void chunk_store(double s, double q, int flag, double
     s += __shfl_xor(s, off);
     q += __shfl_xor(q, off);
   }
-  const int any_flag = __any(flag) ? 1 : 0;
+  // flags are bits (SDEMC_FLAG_OVERFLOW, SDEMC_FLAG_JUMP_CSR): OR over the lanes
+  const int any_flag = (__any(flag & SDEMC_FLAG_OVERFLOW) ? SDEMC_FLAG_OVERFLOW : 0) |
+                       (__any(flag & SDEMC_FLAG_JUMP_CSR) ? SDEMC_FLAG_JUMP_CSR : 0);
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     partial2[0] = s;
     partial2[1] = q;
@@ -829,7 +831,7 @@ __device__ __forceinline__ void chunk_store(double s, double q, int flag, double
 // Fixed-order reduction of chunk partials into {sum, sumsq, n, flags}: thread
 // t takes chunks t, t+256, ...; then a 256-leaf LDS tree.
 __device__ __forceinline__ void finalize_block(const double* partials, const int32_t* cflags, int64_t base,
-                               int64_t nc, uint64_t n, double* st) {
+                               int64_t nc, uint64_t n, double* st, int extra_flags = 0) {
   __shared__ double ss[256];
   __shared__ double qs[256];
   __shared__ int fs[256];
@@ -857,7 +859,7 @@ __device__ __forceinline__ void finalize_block(const double* partials, const int
     st[0] = ss[0];
     st[1] = qs[0];
     st[2] = (double)n;
-    st[3] = (double)fs[0];
+    st[3] = (double)(fs[0] | extra_flags);
   }
 }
 

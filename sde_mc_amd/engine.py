@@ -132,13 +132,20 @@ def simulate_fixed(job, fixed, device, out=None, stats=None):
 
 
 def check_flags(flags, where):
+    """Raise SdemcError for a statistics row whose flags word is not 0."""
+    if flags != flags:  # NaN: no finalisation wrote this row
+        raise L.SdemcError(L.ERR_ARG, where + ' (statistics never written)')
     f = int(flags)
-    if f & 1:
+    if f & L.FLAG_TABLE:
+        raise L.SdemcError(L.ERR_ARG, where + ' (segment table does not match the job table)')
+    if f & L.FLAG_OVERFLOW:
         raise L.SdemcError(L.ERR_OVERFLOW, where)
-    if f & 2:
+    if f & L.FLAG_JUMP_CSR:
         raise L.SdemcError(L.ERR_ARG, where + ' (jump CSR read outside jump_size)')
-    if f & 4:
+    if f & L.FLAG_CLASS:
         raise L.SdemcError(L.ERR_UNSUPPORTED, where + ' (job outside the launch class set)')
+    if f:
+        raise L.SdemcError(L.ERR_ARG, where + ' (unknown flags %d)' % f)
 
 
 def stats_to_host(stats, where='simulate'):

@@ -87,6 +87,15 @@ class _Active:
     __slots__ = ('req', 'job', 'slot', 'base', 'nc', 'next', 'cost', 'admitted', 'in_table')
 
 
+def server_tune(tune, tile_chunks=1):
+    """The launch shape of served rounds: `tune` (default BatchTune()) with
+    `tile_chunks` chunks per tile; every other field, the step unroll
+    included, as tuned."""
+    t = tune or BatchTune()
+    return BatchTune(t.waves_per_block, t.blocks_per_cu, tile_chunks, t.concurrent_groups,
+                     t.step_unroll)
+
+
 class PricingServer:
     """Admission by slots and HBM budget, rounds of batched launches.
 
@@ -107,8 +116,7 @@ class PricingServer:
         self.pool = ChunkPool(hbm_budget_bytes // PARTIAL_BYTES)
         # rounds are a few ms long, so tiles stay one chunk (a long tile at the
         # end of a round leaves the rest of the device idle)
-        t = tune or BatchTune()
-        t = BatchTune(t.waves_per_block, t.blocks_per_cu, tile_chunks, t.concurrent_groups)
+        t = server_tune(tune, tile_chunks)
         self.runner = BatchRunner(max_jobs=max_slots, max_segments=2 * max_slots,
                                   partial_chunks=self.pool.capacity, tune=t, n_buffers=2)
         self.max_slots = max_slots

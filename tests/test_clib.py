@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_layout():
     lib = L.load()
-    assert lib.sdemc_abi_version() == L.ABI_VERSION == 3
+    assert lib.sdemc_abi_version() == L.ABI_VERSION == 4
     L.check_layout()
     assert lib.sdemc_sizeof_prepared_job() % 8 == 0
 
@@ -102,3 +102,79 @@ def test_simulate_rejects_before_touching_device():
     st = lib.sdemc_simulate(ctypes.byref(job), ctypes.c_void_p(8), None, ctypes.c_void_p(8),
                             1 << 20, 3, None)
     assert st == L.ERR_ARG  # waves_per_block must be 1, 2, 4 or 8
+
+
+# ---- host check of a batched launch's tables (sdemc_batch_check) -----------
+def _batch_tables():
+    """A real plan of four jobs of three classes and the prepared slots."""
+    import numpy as np
+    from sde_mc_amd.batch import chunks_of, job_cost, plan_segments
+    lib = L.load()
+    jobs = [J.job_for(J.merton_sde(), 1.0, 100, 5000),
+            J.job_for(J.merton_sde(), 1.0, 50, 300),
+            J.job_for(J.levy_rainbow_sde(), 1.0, 64, 2000),
+            J.job_for(J.merton_sde(), 1.0, 64, 1000, coarse_factor=2)]
+    n = len(jobs)
+    counts = np.array([chunks_of(j.n_paths) for j in jobs], dtype=np.int64)
+    base = np.zeros(n, dtype=np.int64)
+    base[1:] = np.cumsum(counts)[:-1]
+    kb = lib.sdemc_sizeof_prepared_job()
+    max_jobs = 8
+    prepared = ctypes.create_string_buffer(max_jobs * kb)
+    cls = (ctypes.c_int32 * n)()
+    arr = (L.Job * n)(*jobs)
+    assert lib.sdemc_prepare_jobs(arr, n, (ctypes.c_int64 * n)(*base.tolist()), prepared, cls) == L.OK
+    plan = plan_segments(np.arange(n), np.zeros(n, np.int64), counts,
+                         np.array([job_cost(j) for j in jobs]), 2, list(cls))
+    return prepared, max_jobs, plan, int(counts.sum())
+
+
+def _check(prepared, max_jobs, segs, hdr, groups, partial_chunks, max_segments=64):
+    import numpy as np
+    lib = L.load()
+    s = np.zeros((max_segments, L.SEGMENT_INTS), dtype=np.int32)
+    s[:len(segs)] = segs
+    h = np.ascontiguousarray(hdr, dtype=np.int32)
+    g = (ctypes.c_int32 * len(groups))(*groups)
+    return lib.sdemc_batch_check(prepared, max_jobs, s.ctypes.data, max_segments, h.ctypes.data,
+                                 len(groups), g, partial_chunks)
+
+
+def test_batch_check_accepts_a_planned_round():
+    prepared, mj, plan, chunks = _batch_tables()
+    assert len(plan.groups) == 3
+    assert _check(prepared, mj, plan.segs, plan.hdr, plan.groups, chunks) == L.OK
+    # every job is also valid in the generic kernel's group
+    from sde_mc_amd.batch import plan_segments
+    import numpy as np
+    segs = plan.segs.copy()
+    n = len(segs)
+    gen = plan_segments(segs[:, 0], segs[:, 1], segs[:, 2], np.ones(n), 1)
+    assert _check(prepared, mj, gen.segs, gen.hdr, gen.groups, chunks) == L.OK
+
+
+def _mutations():
+    def seg(i, col, v):
+        return lambda s, h, g, pc: (s.__setitem__((i, col), v), (s, h, g, pc))[1]
+    return [
+        ('unprepared slot', seg(0, 0, 5)),
+        ('slot beyond the table', seg(0, 0, 8)),
+        ('negative first chunk', seg(0, 1, -1)),
+        ('chunks past the job', lambda s, h, g, pc: (s.__setitem__((0, 2), s[0, 2] + 1), (s, h, g, pc))[1]),
+        ('zero chunks per tile', seg(0, 3, 0)),
+        ('tile gap', lambda s, h, g, pc: (s.__setitem__((1, 4), s[1, 4] + 1), (s, h, g, pc))[1]),
+        ('tile overlap', lambda s, h, g, pc: (s.__setitem__((1, 4), s[1, 4] - 1), (s, h, g, pc))[1]),
+        ('header tile count', lambda s, h, g, pc: (h.__setitem__((0, 2), h[0, 2] + 1), (s, h, g, pc))[1]),
+        ('header past the list', lambda s, h, g, pc: (h.__setitem__((0, 0), 63), (s, h, g, pc))[1]),
+        ('wrong class', lambda s, h, g, pc: (s, h, tuple(reversed(g)), pc)),
+        ('partial pool too small', lambda s, h, g, pc: (s, h, g, pc - 1)),
+    ]
+
+
+@pytest.mark.parametrize("name,mut", _mutations(), ids=[m[0] for m in _mutations()])
+def test_batch_check_rejects_misplanned_tables(name, mut):
+    prepared, mj, plan, chunks = _batch_tables()
+    # the first group must hold at least two segments for the tile mutations
+    assert plan.hdr[0, 1] >= 2
+    s, h, g, pc = mut(plan.segs.copy(), plan.hdr.copy(), plan.groups, chunks)
+    assert _check(prepared, mj, s, h, g, pc) == L.ERR_ARG, name

@@ -128,3 +128,74 @@ def test_autotune_picks_a_candidate_and_it_prices_the_same():
     keys = [c.key() for c in cands3]
     assert tune2.key() in keys + [k[:3] + (1 - k[3],) + k[4:] for k in keys]
     assert np.array_equal(BatchRunner(max_jobs=16, tune=tune2).run(jobs), plain)
+
+
+def _misplanned_round(mutate):
+    """Run three jobs once (slots 0-2 prepared), then launch a round whose
+    segment table `mutate` spoiled, staged without the host check; returns
+    the finalised statistics [3, 4] and whether the host check refused it."""
+    from sde_mc_amd.batch import NS, chunks_of, job_cost, plan_segments
+    jobs = [J.job_for(J.merton_sde(), 1.0, 32, 5000, option=sm.EuroCall(1.0), seed=61),
+            J.job_for(J.merton_sde(), 1.0, 16, 700, option=sm.EuroCall(1.0), seed=62),
+            J.job_for(J.merton_sde(), 1.0, 24, 3000, option=sm.EuroCall(0.9), seed=63)]
+    r = BatchRunner(max_jobs=8, use_graph=False)
+    good = r.run(jobs)
+    counts = np.array([chunks_of(j.n_paths) for j in jobs], dtype=np.int64)
+    plan = plan_segments(np.arange(3), np.zeros(3, np.int64), counts,
+                         np.array([job_cost(j) for j in jobs]), 1, r.slot_class[:3],
+                         min_tile_cost=1.0)
+    assert plan.groups == (0,) and plan.hdr[0, 1] == 3
+    mutate(plan)
+    buf = r.bufs[0]
+    n = len(plan.segs)
+    buf.h_segs.numpy()[:NS * n] = plan.segs.reshape(-1)
+    buf.h_hdr.numpy()[:] = plan.hdr.reshape(-1)
+    try:
+        r.check_tables(buf, plan.groups)
+        refused = False
+    except L.SdemcError:
+        refused = True
+    buf.segs[:NS * n].copy_(buf.h_segs[:NS * n])
+    buf.hdr.copy_(buf.h_hdr)
+    buf.h_final.numpy()[:] = 0
+    buf.h_final.numpy()[:3] = 1
+    buf.final.copy_(buf.h_final)
+    print('misplanned round: segments %s header %s' % (plan.segs.tolist(), plan.hdr[0].tolist()),
+          flush=True)
+    r.launch(buf, plan.groups)
+    torch.cuda.synchronize()
+    return good, r.stats[:12].cpu().numpy().reshape(3, 4), refused, (r, jobs)
+
+
+def _set(row, col, delta):
+    def f(plan):
+        plan.segs[row, col] += delta
+    return f
+
+
+@pytest.mark.parametrize("name,mutate", [
+    ('unprepared slot', lambda p: p.segs.__setitem__((1, 0), 5)),
+    ('chunks past the job', _set(0, 2, 1)),
+    ('tile ranges overlap', _set(2, 4, -1)),
+    ('tile count short', lambda p: p.hdr.__setitem__((0, 2), p.hdr[0, 2] - 1)),
+])
+def test_misplanned_round_is_flagged_not_stale(name, mutate):
+    # a segment table that does not describe the job table is refused by the
+    # host check before upload and, launched anyway, runs only in-bounds work
+    # and flags every slot it finalises: no round returns the previous
+    # round's statistics as its result
+    good, st, refused, _ = _misplanned_round(mutate)
+    assert refused, name
+    assert np.all(good[:, 3] == 0)
+    for i in range(3):
+        assert int(st[i, 3]) & L.FLAG_TABLE, (name, i, st[i])
+        with pytest.raises(L.SdemcError):
+            E.check_flags(st[i, 3], 'slot %d' % i)
+
+
+def test_wellplanned_round_after_a_misplanned_one_is_clean():
+    # the fault word is per launch: the next good round carries no flag
+    good, st, _, (r, jobs) = _misplanned_round(_set(0, 2, 1))
+    assert int(st[0, 3]) & L.FLAG_TABLE
+    again = r.run(jobs)
+    assert np.all(again[:, 3] == 0) and np.array_equal(again, good)

@@ -172,6 +172,61 @@ def test_rng_paths_match_oracle(model, grid, coupled):
     assert s[2] == P
 
 
+# the Levy rainbow exactly as it is benchmarked and served (jump scale f = 0.2
+# per asset, sde_mc problem.py:158; eps = 0.05, so lambda ~ 25 per year): per
+# step counts at 200 steps (lambda h ~ 0.12), the arrival clock at 800 (~0.03)
+BENCH_LEVY = [('uniform', 200, 0), ('uniform', 800, 0), ('uniform', 200, 2),
+              ('adapted', 200, 0), ('adapted', 200, 2)]
+
+
+@pytest.mark.parametrize("grid,N,coupled", BENCH_LEVY)
+def test_rng_paths_of_the_benchmarked_levy_model(grid, N, coupled):
+    from sde_mc_amd import workloads as W
+    P = 1000
+    sde = W.levy_rainbow()
+    job = J.job_for(sde, 1.0, N, P, grid=grid, coarse_factor=coupled,
+                    option=Rainbow(1.0, log=True, discount=math.exp(0.02)),
+                    disc=math.exp(-0.02), seed=4242, path_offset=77)
+    assert abs(job.model.j1[0]) < 1e-12 and abs(job.model.j0[0] - 0.2) < 1e-12
+    if grid == 'uniform' and not coupled:
+        assert (oracle.step_jumps(job, 0, 0)[0] < 0) == (N == 800)
+    pay, term, ns, s = _run_rng(job)
+    opay, oterm, ons = oracle.simulate(job, 0, P)
+    assert _close(term, oterm) < TOL
+    if grid == 'adapted':
+        assert np.array_equal(ns, ons)
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+    assert s[2] == P and s[3] == 0
+
+
+def test_fixed_benchmarked_levy_model():
+    from sde_mc_amd import workloads as W
+    P, N = 4096 + 3, 200
+    job = J.job_for(W.levy_rainbow(), 1.0, N, P,
+                    option=Rainbow(1.0, log=True, discount=math.exp(0.02)), disc=math.exp(-0.02))
+    fx = J.fixed_uniform(job, seed=13)
+    pay, term, _, s = _run_fixed(job, fx)
+    opay, oterm, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(term, oterm) < TOL
+    assert abs(s[0] / P - opay.mean()) < 1e-5
+
+
+def test_fixed_jump_reads_outside_the_marks_raise():
+    # counts that claim more marks than jump_size holds: the kernel refuses
+    # the reads past n_marks (in bounds, mark 0) and flags the job, which the
+    # host reports as an argument error instead of a price
+    P, N = 256, 20
+    job = J.job_for(J.merton_sde(rate=4.0), 1.0, N, P, option=EuroCall(1.0))
+    fx = J.fixed_uniform(job, seed=14)
+    assert fx.a['js'].size > 8
+    fx.a['js'] = fx.a['js'][:8].copy()
+    print('fixed job with 8 marks for %d jumps' % int(fx.a['counts'].sum()), flush=True)
+    stats = E.simulate_fixed(job, fx.device_struct(_dev()), _dev())
+    with pytest.raises(L.SdemcError) as e:
+        E.stats_to_host(stats, 'short jump_size')
+    assert e.value.status == L.ERR_ARG and 'jump CSR' in str(e.value)
+
+
 @pytest.mark.parametrize("coupled", [0, 2])
 def test_adapted_low_rate_long_refill_windows(coupled):
     """lambda h = 0.0015: the adapted grid refills the second jump slots only

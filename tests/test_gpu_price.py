@@ -105,3 +105,85 @@ def test_mc_multilevel_matches_closed_form():
     # batched and per-level launches compute the same jobs
     assert a.sample_mean == b.sample_mean and a.sample_std == b.sample_std
     assert abs(a.sample_mean - J.MERTON_PRICE) < 4 * a.sample_std + 2e-4
+
+
+def test_run_mc_reaches_the_half_width():
+    # find_num_trials / run_mc (mc.py:418-440): a pilot sizes the run so that
+    # the 95% half-width is eps; the priced call then sits within it (plus
+    # the 200-step Euler bias, ~0.018 T h) of the closed form
+    prob = sm.MertonEuroCall(0.02, 0.2, 2.0, -0.1, 0.15, 1.0, 1.0, 1.0, 200, 'cpu')
+    n = sm.find_num_trials(prob, 1e-3, init_trials=1 << 17, bs=1 << 17)
+    assert 1.4e5 < n < 1.9e5          # (1.96 * 0.207 / 1e-3)^2 ~ 1.65e5
+    stats = sm.run_mc(prob, 1e-3, bs=1 << 18, init_trials=1 << 17)
+    print('run_mc: %d trials, %.6f +- %.6f (half-width %.2e)'
+          % (stats.num_trials, stats.sample_mean, stats.sample_std, 1.96 * stats.sample_std))
+    assert 1.96 * stats.sample_std < 1.03e-3
+    assert abs(stats.sample_mean - J.MERTON_PRICE) < 4 * stats.sample_std + 0.018 * 0.005
+
+
+def test_optimal_trials_reach_the_mlmc_target():
+    # get_optimal_trials (mlmc.py:77-97) from pilot variances of levels 4..64,
+    # then mc_multilevel with those trials: the estimate's standard error
+    # meets eps / 1.96 and the price is within it of the closed form (plus
+    # the finest level's Euler bias, ~0.018 h = 2.8e-4 at 64 steps)
+    prob = sm.MertonEuroCall(0.02, 0.2, 2.0, -0.1, 0.15, 1.0, 1.0, 1.0, 4, 'cpu', grid='uniform')
+    levels = [4, 8, 16, 32, 64]
+    eps = 2e-3
+    trials = sm.get_optimal_trials(1 << 15, levels, eps, prob.solver, prob.payoff, prob.discounter)
+    assert trials[0] > trials[1] > trials[-1] > 0
+    st = sm.mc_multilevel(trials, levels, prob.solver, prob.payoff, prob.discounter)
+    print('mlmc trials %s: %.6f +- %.6f' % (trials, st.sample_mean, st.sample_std))
+    assert st.sample_std < 1.05 * eps / 1.96
+    assert abs(st.sample_mean - J.MERTON_PRICE) < 4 * st.sample_std + 0.018 / 64
+
+
+def _preset_job(prob, n, option, disc, offset=0):
+    job, in_kernel = prob.solver.make_job(n, option=option, discount=disc, path_offset=offset)
+    assert in_kernel or option is None
+    return job
+
+
+def test_levy_call_on_max_preset_on_the_device():
+    # LevyCallOnMax.default_params(2, ...) (problem.py:140-166): alpha 0.5,
+    # tempered tails mu = 2, eps = 0.001 (~124 jumps a year), T = 3, on the
+    # jump-adapted grid.  Paths follow the oracle; each asset's discounted
+    # forward is a martingale, which pins the preset's compensator; the call
+    # on the max lies between the larger single-asset call and their sum.
+    import oracle
+    from tests.test_gpu_parity import TOL, _close, _run_rng
+    prob = sm.LevyCallOnMax.default_params(2, 64, 'cpu')
+    disc = float(prob.discounter(3.0))
+    job = _preset_job(prob, 512, prob.payoff, disc)
+    pay, term, ns, s = _run_rng(job)
+    opay, oterm, ons = oracle.simulate(job, 0, 512)
+    assert np.array_equal(ns, ons) and ns.min() > 64 + 200
+    assert _close(term, oterm) < TOL
+    assert abs(s[0] / 512 - opay.mean()) < 1e-5
+    n = 1 << 19
+    m_call, se_call = _price(_preset_job(prob, n, prob.payoff, disc, offset=1 << 30))
+    singles = []
+    for i in range(2):
+        job = _preset_job(prob, n, None, 1.0, offset=(2 + i) << 30)
+        job.payoff.kind = sm._lib.PAYOFF_IDENTITY
+        job.payoff.log = 1
+        if i == 1:
+            mm = job.model
+            for f in ('x0', 'a0', 'a1', 'b0', 'b1', 'c0', 'c1', 'j0', 'j1'):
+                arr = getattr(mm, f)
+                arr[0], arr[1] = arr[1], arr[0]
+        m, se = _price(job)
+        print('LevyCallOnMax asset %d: E[exp(X_T)] = %.6f +- %.6f' % (i, m, se))
+        assert abs(m - 1.0) < 4 * se + 1e-4
+        call = _preset_job(prob, n, sm.EuroCall(1.0, log=True, discount=math.exp(0.06)), disc,
+                           offset=(4 + i) << 30)
+        if i == 1:
+            mm = call.model
+            for f in ('x0', 'a0', 'a1', 'b0', 'b1', 'c0', 'c1', 'j0', 'j1'):
+                arr = getattr(mm, f)
+                arr[0], arr[1] = arr[1], arr[0]
+        singles.append(_price(call))
+    print('LevyCallOnMax: %.6f +- %.6f, single calls %s' % (m_call, se_call, singles))
+    lo = max(c for c, _ in singles)
+    hi = sum(c for c, _ in singles)
+    tol = 4 * (se_call + max(e for _, e in singles))
+    assert lo - tol < m_call < hi + tol

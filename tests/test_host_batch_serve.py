@@ -210,3 +210,49 @@ def test_autotune_prefers_in_turn_groups_within_the_margin(monkeypatch):
     # the confirm stage's times decide, not the sweep's
     assert pick([10.0, 10.3, 11.0], [10.0, 10.1, 12.0]) == turn.key()
     assert pick([10.0, 10.1, 11.0], [10.0, 10.5, 12.0]) == side.key()
+
+
+def test_server_keeps_the_tuned_step_unroll():
+    # served rounds run the tuned shape with one-chunk tiles; the step unroll
+    # (and every other field) is the tune's
+    from sde_mc_amd.batch import BatchTune
+    from sde_mc_amd.serve import server_tune
+    t = server_tune(BatchTune(2, 16, 2, 0, 2))
+    assert t.key() == (2, 16, 1, 0, 2)
+    assert server_tune(BatchTune(4, 7, 1, 1, 1), tile_chunks=2).key() == (4, 7, 2, 1, 1)
+    assert server_tune(None).key() == BatchTune().key()[:2] + (1,) + BatchTune().key()[3:]
+
+
+@pytest.mark.parametrize("flags,status,text", [
+    (L.FLAG_OVERFLOW, L.ERR_OVERFLOW, ''),
+    (L.FLAG_JUMP_CSR, L.ERR_ARG, 'jump CSR'),
+    (L.FLAG_CLASS, L.ERR_UNSUPPORTED, 'class'),
+    (L.FLAG_TABLE, L.ERR_ARG, 'segment table'),
+    (L.FLAG_TABLE | L.FLAG_OVERFLOW, L.ERR_ARG, 'segment table'),
+    (float('nan'), L.ERR_ARG, 'never written'),
+    (64.0, L.ERR_ARG, 'unknown'),
+])
+def test_statistics_flags_raise(flags, status, text):
+    from sde_mc_amd import engine as E
+    with pytest.raises(L.SdemcError) as e:
+        E.check_flags(flags, 'job')
+    assert e.value.status == status and text in str(e.value)
+    E.check_flags(0.0, 'job')
+
+
+def test_autotune_cache_key_follows_the_build(monkeypatch, tmp_path):
+    # a cached pick is keyed by the ABI version, the built library and the
+    # candidate list: a rebuild or a new candidate list retunes
+    import os
+    from sde_mc_amd import autotune as A
+    from sde_mc_amd.batch import BatchTune
+    lib = tmp_path / 'libsdemc.so'
+    lib.write_bytes(b'x' * 10)
+    monkeypatch.setattr(L, 'lib_path', lambda: str(lib))
+    k1 = A._build_key(A.CANDIDATES)
+    assert 'abi%d' % L.ABI_VERSION in k1
+    os.utime(lib, (1, 12345))
+    k2 = A._build_key(A.CANDIDATES)
+    assert k2 != k1
+    assert A._build_key(A.CANDIDATES[:5]) != k2
+    assert A._build_key([BatchTune(2, 16, 1, 0, 1)]) != A._build_key([BatchTune(2, 16, 1, 0, 2)])

@@ -171,6 +171,72 @@ def test_adapted_matches_torch_fp32_restatement():
     assert np.allclose(paths[:, total, 0].numpy(), term[0, 0], rtol=2e-5)
 
 
+def _coupled_adapted_case(sde, N, M, P, seed, exact_jumps=False):
+    job = J.job_for(sde, 1.0, N, P, grid='adapted', coarse_factor=M, exact_jumps=exact_jumps)
+    fx = J.fixed_adapted(job, seed=seed)
+    _, term, ns = oracle.simulate_fixed(job, fx.host_struct(), P)
+    m = refcpu.model_from_kernel(job.model)
+    fixed = (fx.a['dw'], fx.a['jt'], fx.a['js'])
+    return job, term, ns, m, fixed
+
+
+COUPLED_ADAPTED = [
+    ('merton_old_state', lambda: J.merton_sde(rate=6.0), 24, 2, False),
+    ('merton_exact', lambda: J.merton_sde(rate=6.0), 24, 2, True),
+    ('merton_m4', lambda: J.merton_sde(rate=9.0), 32, 4, False),
+    ('levy_indep', lambda: J.levy_rainbow_sde(eps=0.2), 16, 2, False),
+    ('levy_indep_exact', lambda: J.levy_rainbow_sde(f=(0.2, 0.2)), 16, 2, True),
+]
+
+
+@pytest.mark.parametrize("name,mk,N,M,exact", COUPLED_ADAPTED, ids=[c[0] for c in COUPLED_ADAPTED])
+def test_adapted_coupled_matches_torch_fp32_restatement(name, mk, N, M, exact):
+    # the C oracle's coupled jump-adapted pair equals the op-for-op torch-fp32
+    # restatement of multilevel_solve (solvers.py:228-307) on the same fixed
+    # draws: identical iteration counts per path, fine and coarse states
+    # within fp32
+    P = 64
+    job, term, ns, m, fixed = _coupled_adapted_case(mk(), N, M, P, seed=21, exact_jumps=exact)
+    xf, xc, iters = refcpu.multilevel_adapted(m, 1.0, N, N // M, P, fixed, exact_jumps=exact)
+    assert np.array_equal(iters.numpy(), ns)
+    # jumps happened inside the window, several paths took event steps
+    assert (ns > N // M).sum() >= P // 4
+    for i in range(job.model.dim):
+        assert np.allclose(xf[:, i].numpy(), term[0, i], rtol=2e-5, atol=2e-6), (name, 'fine', i)
+        assert np.allclose(xc[:, i].numpy(), term[1, i], rtol=2e-5, atol=2e-6), (name, 'coarse', i)
+
+
+@pytest.mark.parametrize("plant", ['coarse_post_state', 'coarse_untruncated', 'sum_unscaled'])
+def test_adapted_coupled_restatement_detects_planted_mistakes(plant):
+    # the comparison above is sharp enough to see each plausible mistake of a
+    # coupled jump-adapted step: a loop with the mistake planted disagrees
+    # with the oracle on the coarse path
+    P, N, M = 64, 24, 2
+    job, term, ns, m, fixed = _coupled_adapted_case(J.merton_sde(rate=6.0), N, M, P, seed=21)
+    xf, xc, _ = refcpu.multilevel_adapted(m, 1.0, N, N // M, P, fixed, plant=plant)
+    assert np.allclose(xf[:, 0].numpy(), term[0, 0], rtol=2e-5)
+    assert not np.allclose(xc[:, 0].numpy(), term[1, 0], rtol=2e-5, atol=2e-6)
+
+
+def test_oracle_mlmc_telescopes_adapted():
+    # on the jump-adapted grid too, E[P_fine - P_coarse] from the coupled
+    # pair equals the difference of the two single-level adapted prices, and
+    # the correction's variance is far below a level's
+    n = 1 << 14
+    sde = J.merton_sde(rate=4.0)
+    opt, disc = EuroCall(1.0), math.exp(-0.02)
+    c = J.job_for(sde, 1.0, 8, n, grid='adapted', coarse_factor=2, option=opt, disc=disc)
+    d, _, _ = oracle.simulate(c, 0, n)
+    f = J.job_for(sde, 1.0, 8, n, grid='adapted', option=opt, disc=disc, seed=11)
+    g = J.job_for(sde, 1.0, 4, n, grid='adapted', option=opt, disc=disc, seed=12)
+    pf, _, _ = oracle.simulate(f, 0, n)
+    pg, _, _ = oracle.simulate(g, 0, n)
+    diff = pf.mean() - pg.mean()
+    se = math.sqrt(pf.var() / n + pg.var() / n + d.var() / n)
+    assert abs(d.mean() - diff) < 4 * se
+    assert d.var() < 0.1 * pf.var()
+
+
 def test_coupled_coarse_equals_single_on_summed_increments():
     # the coarse path of a coupled pair is a single-level run on the coarse
     # grid whose increments are pairwise sums and whose jumps are the fine ones
