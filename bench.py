@@ -24,13 +24,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "path-steps/s batched vs plain at equal CI; p50/p99 pricing latency"
 PEAK_VALU_OPS = 78.6e12                   # fp32 vector FMA lane-ops/s (157.3 TFLOP/s / 2)
-PEAK_FP32 = 157.3e12
+PEAK_FP32 = 157.3e12                      # MI355X fp32 vector peak, FLOP/s
 
 
-def cpu_baseline(reqs, budget_s=12.0, cap_paths=1 << 12):
+def cpu_baseline(reqs, budget_s=6.0, cap_paths=1 << 12):
     """The reference's solver loop restated op for op in torch on the host
     cores (oracle/refcpu.py), over the mix's requests in order with each
-    request's path count capped; path-steps/s of the sample."""
+    request's path count capped; path-steps/s of the sample (a second CPU
+    figure beside the planned 2^16 x 100 Merton baseline, c0_baseline)."""
     import torch
     from oracle import refcpu
     from sde_mc_amd import engine as E
@@ -70,15 +71,211 @@ def cpu_baseline(reqs, budget_s=12.0, cap_paths=1 << 12):
 
 MERTON_CLOSED_FORM = 0.13205167640082938  # S0=K=1, r=0.02, sigma=0.2, lambda=2, N(-0.1, 0.15^2), T=1
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
-# each scaled by the counter's reading on a 1 GiB device copy (fetch x2.0,
-# write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
-# runs.  Served mix round (k_batch group kernels, 128 requests, seed 0),
-# keyed by the launch shape's chunks per tile: nearly all of it is the chunk
-# partials' writes, which scatter more with one chunk per tile; and the
-# fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = {1: 1.134e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
-TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
+
+def _median(v):
+    v = sorted(v)
+    n = len(v)
+    return v[n // 2] if n % 2 else 0.5 * (v[n // 2 - 1] + v[n // 2])
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown"
+
+
+def c0_baseline(dev, runs=5, P=1 << 16, N=100, T=1.0):
+    """The plain method on the CPU, as planned for the 1-D Merton call at 2^16
+    paths x 100 steps, seed 0: the reference's jump-adapted solver loop with
+    path storage and mc_simple's unbatched payoff (solvers.py:164-226,
+    mc.py:87-100) restated op for op in torch fp32 (oracle/refcpu.py), on the
+    host cores this process uses; 1 warm-up, median of `runs`.  Beside it the
+    GPU's plain kernel on the same workload (same model, grid, paths and
+    steps; its own Philox stream), median of `runs` event-timed launches."""
+    import torch
+    from scipy.stats import poisson
+    from oracle import refcpu
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import workloads as W
+    from sde_mc_amd.options import EuroCall
+    lam = 2.0
+    max_jumps = max(int(T * poisson.ppf(1 - 1 / 1e9, lam)), 5)  # solvers.py:133
+    model = refcpu.merton_model(0.02, 0.2, lam, -0.1, 0.15)
+    disc = math.exp(-0.02 * T)
+    payoff = lambda x: torch.clamp(x[:, 0] - 1.0, min=0.0)
+    gen = torch.Generator().manual_seed(0)
+    refcpu.price(model, payoff, disc, T, N, P, max_jumps, gen)          # warm-up
+    times, prices = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        mean, se = refcpu.price(model, payoff, disc, T, N, P, max_jumps, gen)
+        times.append(time.perf_counter() - t0)
+        prices.append((mean, se))
+    t = _median(times)
+    mean, se = prices[times.index(t)] if t in times else prices[0]
+    cpu = {"value": P * N / t, "unit": "path-steps/s", "cores": torch.get_num_threads(),
+           "affinity_cpus": len(os.sched_getaffinity(0)), "cpu": _cpu_model(), "kind": "port",
+           "sample": "the 1-D Merton call (S0=K=1, r=0.02, sigma=0.2, lambda=2, log-jump "
+                     "N(-0.1, 0.15^2), T=1) at %d paths x %d steps, seed 0: the reference's "
+                     "jump-adapted loop op for op in torch fp32 with path storage, 1 warm-up, "
+                     "median of %d" % (P, N, runs),
+           "runs_s": times, "price": mean, "half_width": 1.96 * se,
+           "error_vs_closed_form": mean - MERTON_CLOSED_FORM}
+    gpu = {}
+    for grid, code in (("adapted", L.GRID_ADAPTED), ("uniform", L.GRID_UNIFORM)):
+        job = E.make_job(W.model_description('merton'), T, N, code, P, 0, 0,
+                         max_steps=N + E.safe_max_jumps(lam, T))
+        E.fill_payoff(job, EuroCall(1.0), disc)
+        stats = E.simulate(job, dev)
+        torch.cuda.synchronize()
+        ms = _median([_time_events(lambda: E.simulate(job, dev, stats=stats), 1)
+                      for _ in range(runs)])
+        s, q, n = E.stats_to_host(stats, 'C0 on the GPU')
+        m, e = E.mean_and_se(s, q, n)
+        gpu[grid] = {"path_steps_s": P * N / (ms * 1e-3), "kernel_ms": ms, "price": m,
+                     "half_width": 1.96 * e, "error_vs_closed_form": m - MERTON_CLOSED_FORM}
+    cpu["gpu_plain_same_workload"] = gpu
+    cpu["gpu_over_cpu_adapted"] = gpu["adapted"]["path_steps_s"] / cpu["value"]
+    return cpu
+
+
+def _sample_job(job, n):
+    """The first n paths of `job` (same seed and path offsets, so the same
+    Philox streams as those paths in the full job)."""
+    import ctypes
+    from sde_mc_amd import _lib as L
+    j = L.Job.from_buffer_copy(job)
+    j.n_paths = min(int(n), int(job.n_paths))
+    return j
+
+
+def _fp64_error(dev, jobs, n):
+    """fp32 kernel against the fp64 oracle on the first n paths of each job:
+    max and mean |x32 - x64| / (1 + |x64|) over terminal states, and the
+    difference of the sample means of the discounted payoffs."""
+    import numpy as np
+    import torch
+    import oracle
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    rel_max, rel_sum, cnt, dprice = 0.0, 0.0, 0, []
+    for job in jobs:
+        j = _sample_job(job, n)
+        P = int(j.n_paths)
+        lv = 2 if j.coarse_factor else 1
+        d = j.model.dim
+        term = torch.empty((lv, d, P), dtype=torch.float32, device=dev)
+        pay = torch.empty(P, dtype=torch.float32, device=dev)
+        out = L.PathOut()
+        out.terminal_dev, out.payoff_dev = term.data_ptr(), pay.data_ptr()
+        E.stats_to_host(E.simulate(j, dev, out=out), 'fp64 error sample')
+        opay, oterm, _ = oracle.simulate(j, 0, P)
+        t32 = term.cpu().numpy().astype(np.float64)
+        err = np.abs(t32 - oterm) / (1.0 + np.abs(oterm))
+        rel_max = max(rel_max, float(err.max()))
+        rel_sum += float(err.sum())
+        cnt += err.size
+        dprice.append(abs(float(pay.cpu().double().mean()) - float(opay.mean())))
+    return {"jobs": len(jobs), "paths_per_job": int(n), "max_rel_state_err": rel_max,
+            "mean_rel_state_err": rel_sum / max(cnt, 1), "max_price_diff": max(dprice),
+            "mean_price_diff": float(np.mean(dprice))}
+
+
+def fp64_error_statement(dev, reqs, mlmc_jobs=None, n=256):
+    """The error of each result against fp64, stated per leg: the served
+    mix (every request), the MLMC levels and the fixed-increment mode,
+    recomputed on a sample of paths by the fp64 oracle (the checker; outside
+    every timed region)."""
+    from sde_mc_amd import workloads as W
+    out = {"served_mix": _fp64_error(dev, [W.request_job(r) for r in reqs], n)}
+    if mlmc_jobs:
+        out["mlmc"] = _fp64_error(dev, mlmc_jobs, n)
+    out["fixed_increment"] = _fixed_fp64_error(dev)
+    out["note"] = ("states: max/mean over the sample of |x32 - x64| / (1 + |x64|); prices: "
+                   "|mean of the fp32 kernel's payoffs - mean of the fp64 oracle's| on the "
+                   "same paths; the job sums themselves are fp64")
+    return out
+
+
+def _fixed_fp64_error(dev, P=4096, N=500, seed=1):
+    """Fixed-increment mode against the fp64 oracle on the same host inputs."""
+    import numpy as np
+    import torch
+    import oracle
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    from sde_mc_amd import workloads as W
+    from sde_mc_amd.options import EuroCall
+    rng = np.random.default_rng(seed)
+    h = 1.0 / N
+    dw = (rng.standard_normal((N, 1, P)) * math.sqrt(h)).astype(np.float32)
+    counts = rng.poisson(2.0 * h, size=(N, P)).astype(np.uint8)
+    off = np.zeros(P + 1, dtype=np.int64)
+    off[1:] = np.cumsum(counts.sum(axis=0))
+    js = (np.exp(-0.1 + 0.15 * rng.standard_normal(max(int(off[-1]), 1))) - 1).astype(np.float32)
+    job = E.make_job(W.model_description('merton'), 1.0, N, L.GRID_UNIFORM, P, 0, 0)
+    E.fill_payoff(job, EuroCall(1.0), math.exp(-0.02))
+    arrays = dict(dw=dw, counts=counts, off=off, js=js)
+
+    def fixed_in(ptrs):
+        f = L.FixedIn()
+        f.dw_dev, f.counts_dev, f.jump_off_dev = ptrs['dw'], ptrs['counts'], ptrs['off']
+        f.jump_time_dev, f.jump_size_dev = None, ptrs['js']
+        f.n_marks, f.n_rows, f.max_jumps = int(js.size), N, 0
+        return f
+    dev_t = {k: torch.from_numpy(v).to(dev) for k, v in arrays.items()}
+    term = torch.empty((1, 1, P), dtype=torch.float32, device=dev)
+    pay = torch.empty(P, dtype=torch.float32, device=dev)
+    out = L.PathOut()
+    out.terminal_dev, out.payoff_dev = term.data_ptr(), pay.data_ptr()
+    E.stats_to_host(E.simulate_fixed(job, fixed_in({k: t.data_ptr() for k, t in dev_t.items()}),
+                                     dev, out=out), 'fixed fp64 error sample')
+    opay, oterm, _ = oracle.simulate_fixed(job, fixed_in({k: v.ctypes.data for k, v in arrays.items()}), P)
+    err = np.abs(term.cpu().numpy().astype(np.float64) - oterm) / (1.0 + np.abs(oterm))
+    return {"jobs": 1, "paths_per_job": P, "max_rel_state_err": float(err.max()),
+            "mean_rel_state_err": float(err.mean()),
+            "max_price_diff": abs(float(pay.cpu().double().mean()) - float(opay.mean())),
+            "mean_price_diff": abs(float(pay.cpu().double().mean()) - float(opay.mean()))}
+
+
+def valu_issue_ceiling(dev, iters=8192, reps=3):
+    """Measured VALU issue rates: back-to-back instructions of one kind from 8
+    resident wavefronts per SIMD on every CU (sdemc_valu_probe_dev), wavefront
+    instructions per second, best of `reps` launches each.  Kind 0 (v_fma_f32)
+    is the ceiling the path kernels' VALU instruction counts are priced at;
+    1 (v_xor_b32 + v_add_u32), 2 (v_mad_u64_u32 + v_xor_b32) and 3 (v_exp_f32
+    + 2 v_mul_f32) price Philox and Box-Muller."""
+    import ctypes
+    import torch
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    blocks = cus * 8
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    lib = L.lib()
+    clock_khz, cu, ws = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    lib.sdemc_device_info(torch.cuda.current_device(), ctypes.byref(cu), ctypes.byref(clock_khz),
+                          ctypes.byref(ws))
+    per_iter = {0: 32, 1: 64, 2: 64, 3: 96}   # VALU instructions per loop iteration
+    names = {0: "v_fma_f32", 1: "v_xor_b32 + v_add_u32", 2: "v_mad_u64_u32 + v_xor_b32",
+             3: "v_exp_f32 + 2 v_mul_f32"}
+    rec = {"clock_mhz": clock_khz.value / 1e3, "cus": cus,
+           "probe": "sdemc_valu_probe_dev, %d workgroups of 4 wavefronts, %d iterations" % (blocks, iters)}
+    for kind in (0, 1, 2, 3):
+        def run():
+            L.check(lib.sdemc_valu_probe_dev(ctypes.c_void_p(out.data_ptr()), blocks, iters, kind,
+                                             E.stream_ptr()), 'sdemc_valu_probe_dev')
+        run()
+        torch.cuda.synchronize()
+        ms = min(_time_events(run, 1) for _ in range(reps))
+        rate = blocks * 4 * per_iter[kind] * iters / (ms * 1e-3)
+        rec[names[kind]] = {"wave_instr_per_s": rate, "kernel_ms": ms,
+                            "wave_instr_per_cu_cycle": rate / (cus * clock_khz.value * 1e3)}
+    rec["wave_instr_per_s"] = rec[names[0]]["wave_instr_per_s"]
+    return rec
 
 
 def _time_events(fn, reps):
@@ -213,7 +410,7 @@ def mlmc_leg(dev, reps=5):
     P_l = 2^(22 - 10 l / 6) paths, coupled fine/coarse pairs for l >= 1, all
     levels in one batched launch (launch shape autotuned on the level set,
     a batch of other classes than the served mix) vs one plain launch per
-    level.  Path-steps count fine steps only."""
+    level.  Path-steps count fine steps only.  Returns (record, level jobs)."""
     import numpy as np
     import torch
     from sde_mc_amd import _lib as L
@@ -261,7 +458,7 @@ def mlmc_leg(dev, reps=5):
             "tune": dict(zip(("waves_per_block", "blocks_per_cu", "tile_chunks",
                               "concurrent_groups", "step_unroll"), tune.key())),
             "estimate": est, "rmse_se": math.sqrt(var), "closed_form": MERTON_CLOSED_FORM,
-            "error": est - MERTON_CLOSED_FORM}
+            "error": est - MERTON_CLOSED_FORM}, jobs
 
 
 def _mlp_flops(layers):
@@ -377,6 +574,11 @@ def main():
                     help="run only the control-variate leg and print its record")
     ap.add_argument("--probe-copy-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (calibrates HBM counters)")
+    ap.add_argument("--valu-probe-only", action="store_true",
+                    help="run only the VALU issue-rate probe and print its record")
+    ap.add_argument("--c0-only", action="store_true",
+                    help="run only the CPU baseline on the 2^16 x 100 Merton call (and the GPU "
+                         "plain kernel beside it) and print its record")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X per process and "
@@ -403,10 +605,18 @@ def main():
     if args.cv_only:
         print(json.dumps({"control_variates": cv_leg(dev, args.steps)}))
         return
+    if args.valu_probe_only:
+        print(json.dumps({"valu_issue_ceiling": valu_issue_ceiling(dev)}))
+        return
+    if args.c0_only:
+        print(json.dumps({"cpu_baseline": c0_baseline(dev)}))
+        return
     reqs = W.request_mix(args.requests, seed=args.seed)
     jobs = [W.request_job(r) for r in reqs]
     total_ps = sum(r.path_steps for r in reqs)
     ops = sum(r.path_steps * W.valu_ops_per_path_step(r.model, r.steps, r.maturity) for r in reqs)
+    flops = sum(r.path_steps * W.fp32_flops_per_path_step(r.model, r.steps, r.maturity)
+                for r in reqs)
     chunks = sum(chunks_of(r.paths) for r in reqs)
     # autotune the persistent launch shape on a calibration copy of the mix
     # (same requests, path counts capped at 2^18)
@@ -472,15 +682,24 @@ def main():
                                   "results_identical": bool(same)}
 
     extra = {}
+    mlmc_jobs = None
     if not args.no_extra:
         extra["fixed_increment"] = fixed_increment_leg(dev, args.steps)
         extra["equal_ci"] = equal_ci_leg(reqs, tune, dev)
-        extra["mlmc"] = mlmc_leg(dev)
+        extra["mlmc"], mlmc_jobs = mlmc_leg(dev)
         extra["control_variates"] = cv_leg(dev)
 
-    cpu = None if args.no_cpu else cpu_baseline(reqs)
+    ceiling = valu_issue_ceiling(dev)
+    cpu = None
+    if not args.no_cpu:
+        cpu = c0_baseline(dev)
+        cpu["mix_sample"] = cpu_baseline(reqs)
+        extra["fp64_error"] = fp64_error_statement(dev, reqs, mlmc_jobs)
 
-    achieved = ops / (kern_ms * 1e-3)
+    kern_s = kern_ms * 1e-3
+    achieved = flops / kern_s
+    mix_default = (args.requests, args.seed) == (128, 0)
+    valu_instr = VALU_INSTR_MIX_ROUND if mix_default else None
     out = {
         "metric": METRIC,
         "value": batched,
@@ -506,12 +725,25 @@ def main():
         "plain_value": plain,
         "speedup_vs_plain": (batched / plain) if plain else None,
         "latency": latency,
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12,
-                     "unit": "Top/s (VALU lane-ops, algorithmic)",
-                     "frac": achieved / PEAK_VALU_OPS,
-                     "traffic": (TRAFFIC_MIX_ROUND.get(tune.tile_chunks)
-                                 if (args.requests, args.seed) == (128, 0) else None),
-                     "kernel": "k_batch (persistent tile kernel)", "kernel_ms": kern_ms},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s (fp32 vector, algorithmic: 12 per 1-D Merton path-step, "
+                             "35 per 2-D Levy path-step, + 10 per expected jump)",
+                     "frac": achieved / PEAK_FP32,
+                     "traffic": TRAFFIC_MIX_ROUND.get(tune.tile_chunks) if mix_default else None,
+                     "kernel": "k_batch (persistent tile kernel)", "kernel_ms": kern_ms,
+                     "valu_issue": {
+                         "instr_per_round": valu_instr,
+                         "achieved_wave_instr_per_s": valu_instr / kern_s if valu_instr else None,
+                         "ceiling_wave_instr_per_s": ceiling["wave_instr_per_s"],
+                         "frac": (valu_instr / kern_s / ceiling["wave_instr_per_s"]
+                                  if valu_instr else None),
+                         "source": "SQ_INSTS_VALU of a round (rocprofv3 --pmc) over the live "
+                                   "round time, against the live v_fma_f32 issue probe"},
+                     "lane_ops": {"achieved": ops / kern_s / 1e12, "peak": PEAK_VALU_OPS / 1e12,
+                                  "unit": "Top/s (VALU lane-ops counted from the algorithm, "
+                                          "Philox 20 per word)",
+                                  "frac": ops / kern_s / PEAK_VALU_OPS}},
+        "valu_issue_ceiling": ceiling,
         "cpu_baseline": cpu,
     }
     out.update(extra)

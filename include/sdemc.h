@@ -437,6 +437,16 @@ int sdemc_batch_plan_destroy(sdemc_batch_plan* plan);
 int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint32_t* out_dev,
                      int64_t n, void* stream);
 
+/* VALU issue-rate probes (no counterpart in the reference; they measure the
+ * ceilings the path kernels' roofline is priced at): `blocks` workgroups of
+ * 256 threads each run `iters` iterations of 32 independent operations of one
+ * kind and write one float to out_dev[blocks * 256].  kind 0: fp32 FMA
+ * (v_fma_f32; a launch issues blocks * 4 * 32 * iters wavefront FMAs);
+ * 1: 32-bit integer xor + add; 2: 32 x 32 -> 64-bit multiply; 3: fp32
+ * exp2 and two multiplies. */
+int sdemc_valu_probe_dev(float* out_dev, int32_t blocks, int32_t iters, int32_t kind,
+                         void* stream);
+
 /* Standard normals exactly as the stepping kernels draw them: normal i of
  * path p (i = step * K + k) for p in [0, n_paths), i in [0, n_per_path).
  * out is [n_per_path][n_paths]. */

@@ -156,6 +156,8 @@ _SIGS = {
     "sdemc_batch_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sdemc_philox_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
+    "sdemc_valu_probe_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_void_p]),
     "sdemc_normals_dev": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
                                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
 }

@@ -317,7 +317,11 @@ class BatchRunner:
             self.h_jobs.data_ptr(), self.max_jobs, buf.h_segs.data_ptr(), buf.max_segments,
             buf.h_hdr.data_ptr(), len(groups), gc, self.partial_chunks), 'sdemc_batch_check')
 
-    def stage_round(self, buf, plan, finalize_slots):
+    def stage_round(self, buf, plan, finalize_slots, check=True):
+        """Stage a round's segment list, header and finalise mask into `buf`
+        (pinned host copy, then H2D on the current stream), after
+        sdemc_batch_check of the tables; check=False skips the host check (the
+        device then flags what the check would have refused)."""
         segs = plan.segs
         n = len(segs)
         if n > buf.max_segments:
@@ -325,7 +329,8 @@ class BatchRunner:
         if n:
             buf.h_segs.numpy()[:NS * n] = segs.reshape(-1)
         buf.h_hdr.numpy()[:] = plan.hdr.reshape(-1)
-        self.check_tables(buf, plan.groups)
+        if check:
+            self.check_tables(buf, plan.groups)
         if n:
             buf.segs[:NS * n].copy_(buf.h_segs[:NS * n], non_blocking=True)
         buf.hdr.copy_(buf.h_hdr, non_blocking=True)

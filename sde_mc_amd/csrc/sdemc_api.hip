@@ -134,6 +134,63 @@ __global__ void k_normals(uint32_t k0, uint32_t k1, uint64_t path_offset, int64_
   for (int32_t i = 0; i < n_per_path; ++i) out[(int64_t)i * n_paths + p] = ns.get((uint32_t)i);
 }
 
+// VALU issue-rate probes: 8 independent chains per lane, 4 links of each per
+// loop iteration (the loop counter lives on the scalar unit), so the
+// wavefronts of a full device issue back-to-back instructions of one kind:
+//   KIND 0  v_fma_f32                    x = x a + b
+//   KIND 1  32-bit integer ALU           x = (x ^ a) + b
+//   KIND 2  32 x 32 -> 64-bit multiply   (hi, lo) of x * a, x = hi ^ lo
+//   KIND 3  fp32 transcendental + FMA    x = exp2(x a) b
+// KIND 0 measures the ceiling the path kernels' VALU instruction counts are
+// priced at; the others the costs of Philox (1, 2) and Box-Muller (3).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int32_t iters,
+                                                    float a, float b) {
+  if constexpr (KIND == 0 || KIND == 3) {
+    float x[8];
+    const float s = (float)threadIdx.x * 1e-3f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = s + (float)j;
+    for (int32_t i = 0; i < iters; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if constexpr (KIND == 0) x[j] = __fmaf_rn(x[j], a, b);
+          else x[j] = __builtin_amdgcn_exp2f(x[j] * a) * b;
+        }
+      }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc += x[j];
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+  } else {
+    uint32_t x[8];
+    const uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = threadIdx.x * 2654435761u + (uint32_t)j;
+    for (int32_t i = 0; i < iters; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if constexpr (KIND == 1) {
+            x[j] = (x[j] ^ ua) + ub;
+          } else {
+            const uint64_t m = (uint64_t)x[j] * (uint64_t)ua;
+            x[j] = (uint32_t)(m >> 32) ^ (uint32_t)m;
+          }
+        }
+      }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc ^= x[j];
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = __uint_as_float(acc);
+  }
+}
+
 }  // namespace sdemc
 
 using namespace sdemc;
@@ -348,6 +405,20 @@ int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint
   if (blocks > 0x7fffffffLL) return SDEMC_ERR_ARG;
   hipLaunchKernelGGL(k_philox, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, ctr_dev,
                      key0, key1, out_dev, n);
+  return hip_status(hipGetLastError());
+}
+
+int sdemc_valu_probe_dev(float* out_dev, int32_t blocks, int32_t iters, int32_t kind,
+                         void* stream) {
+  if (!out_dev || blocks < 1 || iters < 1 || kind < 0 || kind > 3) return SDEMC_ERR_ARG;
+  const dim3 g((unsigned)blocks), b(256);
+  const hipStream_t s = (hipStream_t)stream;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(k_valu_probe<0>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
+    case 1: hipLaunchKernelGGL(k_valu_probe<1>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
+    case 2: hipLaunchKernelGGL(k_valu_probe<2>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
+    default: hipLaunchKernelGGL(k_valu_probe<3>, g, b, 0, s, out_dev, iters, 0.001f, 0.999f); break;
+  }
   return hip_status(hipGetLastError());
 }
 

@@ -99,6 +99,21 @@ def request_job(req, grid=L.GRID_UNIFORM):
     return job
 
 
+# Algorithmic fp32 FLOPs per path-step (add, mul, compare, convert = 1, FMA = 2,
+# each transcendental 1), the count the fp32 roofline prices:
+#   1-D Merton: a Box-Muller normal amortised 5, the increment's scale 1, the
+#     Euler update 4, the per-step jump test 2 (12 in all);
+#   2-D Levy rainbow: 3 normals (two correlated Brownian, one common small-jump
+#     Gaussian) 15, the 2x2 correlation 3, their scales 3, two asset updates
+#     with the second driver 12, the jump test 2 (35 in all);
+# plus 10 per expected jump (the mark's draw and its application) and 10 per
+# path for the payoff and the reduction, spread over the steps.
+def fp32_flops_per_path_step(model, steps, T):
+    m = model_description(model) if isinstance(model, str) else model
+    base = 12.0 if m.dim == 1 and m.noise == L.NOISE_DIAG else 35.0
+    return base + 10.0 * m.rate * T / steps + 10.0 / steps
+
+
 # VALU lane-operations per path-step, counted from the algorithm (not from a
 # profile): Philox4x32-10 is 10 rounds of 2 mul-lo + 2 mul-hi + 4 xor per
 # 4 words (key schedule is wave-uniform, on the scalar unit) = 20 per word;

@@ -93,7 +93,8 @@ def test_generic_class_with_dedicated_classes():
 
 def test_class_mismatch_is_flagged_not_silent():
     # a coupled job planned into the single-level kernel's group is refused by
-    # the kernel (NaN partials, flag 4), which the host reports
+    # the host check before upload and, staged anyway, by the kernel (NaN
+    # partials, flag 4), which the host reports
     from sde_mc_amd import _lib as L
     from sde_mc_amd.batch import plan_segments
     job = J.job_for(J.merton_sde(), 1.0, 32, 500, coarse_factor=2, option=sm.EuroCall(1.0))
@@ -102,7 +103,9 @@ def test_class_mismatch_is_flagged_not_silent():
     r.upload_slots([0])
     plan = plan_segments([0], [0], [8], [1.0], 1, [0])
     buf = r.bufs[0]
-    r.stage_round(buf, plan, [0])
+    with pytest.raises(L.SdemcError):
+        r.stage_round(buf, plan, [0])
+    r.stage_round(buf, plan, [0], check=False)
     r.launch(buf, plan.groups)
     r.read_stats(buf, [0])
     buf.done.synchronize()
@@ -134,7 +137,7 @@ def _misplanned_round(mutate):
     """Run three jobs once (slots 0-2 prepared), then launch a round whose
     segment table `mutate` spoiled, staged without the host check; returns
     the finalised statistics [3, 4] and whether the host check refused it."""
-    from sde_mc_amd.batch import NS, chunks_of, job_cost, plan_segments
+    from sde_mc_amd.batch import chunks_of, job_cost, plan_segments
     jobs = [J.job_for(J.merton_sde(), 1.0, 32, 5000, option=sm.EuroCall(1.0), seed=61),
             J.job_for(J.merton_sde(), 1.0, 16, 700, option=sm.EuroCall(1.0), seed=62),
             J.job_for(J.merton_sde(), 1.0, 24, 3000, option=sm.EuroCall(0.9), seed=63)]
@@ -147,19 +150,12 @@ def _misplanned_round(mutate):
     assert plan.groups == (0,) and plan.hdr[0, 1] == 3
     mutate(plan)
     buf = r.bufs[0]
-    n = len(plan.segs)
-    buf.h_segs.numpy()[:NS * n] = plan.segs.reshape(-1)
-    buf.h_hdr.numpy()[:] = plan.hdr.reshape(-1)
     try:
-        r.check_tables(buf, plan.groups)
+        r.stage_round(buf, plan, range(3))
         refused = False
     except L.SdemcError:
         refused = True
-    buf.segs[:NS * n].copy_(buf.h_segs[:NS * n])
-    buf.hdr.copy_(buf.h_hdr)
-    buf.h_final.numpy()[:] = 0
-    buf.h_final.numpy()[:3] = 1
-    buf.final.copy_(buf.h_final)
+    r.stage_round(buf, plan, range(3), check=False)
     print('misplanned round: segments %s header %s' % (plan.segs.tolist(), plan.hdr[0].tolist()),
           flush=True)
     r.launch(buf, plan.groups)
