@@ -278,6 +278,21 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
     return rec
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# each scaled by the counter's reading on a 1 GiB device copy (fetch x2.0,
+# write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
+# runs.  Served mix round (k_batch group kernels, 128 requests, seed 0),
+# keyed by the launch shape's chunks per tile: nearly all of it is the chunk
+# partials' writes, which scatter more with one chunk per tile; and the
+# fixed-increment kernel (2^21 x 500).
+TRAFFIC_MIX_ROUND = {1: 1.134e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
+TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
+# VALU instructions (wavefront-level, SQ_INSTS_VALU summed over the group
+# kernels) of one round of the served mix (128 requests, seed 0), from
+# rocprofv3 --pmc; profiles/k_batch_served_mix_128req_sq_counters.md.
+VALU_INSTR_MIX_ROUND = 3.95e10
+
+
 def _time_events(fn, reps):
     """Mean device time (ms) of fn() over reps, bracketed by events on the
     current stream (the stream the library launches on)."""

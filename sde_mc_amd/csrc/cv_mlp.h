@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_cv_uniform(const KJob job, const uint16
       cv_mlp(0, f_hidden, lane, F, fo);
       cv = __fmaf_rn(fast_exp(-rate * t), cv_dot<D, FAM>(fo, dW, dB), cv);
     }
-    euler<D, FAM>(f, x, h, dW, dB);
+    uniform_step<D, FAM>(job, x, z);
   }
   float v = 0.0f;
   if (valid) {

@@ -218,7 +218,18 @@ struct KJob {
   int32_t clock_win;  // steps per jump-queue fill window (uniform grid), per jump-slot refill (adapted)
   uint32_t pois_T[16];
   double pois_inv[16];  // reciprocal width of count c's interval at [c - 1]
+  // Uniform-grid step form (uniform_step): UFORM_GENERIC, or the step's
+  // constants folded on the host for coefficients that are all constant
+  // (UFORM_CONST: x += uA + uW z + uC z_D) or all proportional to the state
+  // (UFORM_PROP: x += x (uA + uW z + uC z_D)), with uA = a h, uW = diag(b)
+  // L sqrt(h) (lower triangular), uC = c sqrt(h).
+  int32_t uform;
+  int32_t additive;  // every jump coefficient constant (j1 = 0): x += j0 J
+  float uA[SDEMC_MAX_DIM], uC[SDEMC_MAX_DIM];
+  float uW[SDEMC_MAX_DIM][SDEMC_MAX_DIM];
 };
+
+enum { UFORM_GENERIC = 0, UFORM_CONST = 1, UFORM_PROP = 2 };
 
 // Thresholds of the per-step jump count: pois_T[k] = round(P(N >= k+1) 2^32)
 // for N ~ Poisson(mu), mu = rate * T / steps; the tail sums run from the top
@@ -352,6 +363,37 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   if (m.jump_law == SDEMC_JUMP_LEVY && j.grid == SDEMC_GRID_UNIFORM) {
     const double mu = m.rate * j.T / (double)j.steps;
     if (mu >= POIS_MIN_MU) k.pois_mode = poisson_thresholds(mu, k.pois_T, k.pois_inv) ? 1 : 0;
+  }
+  // Folded uniform-grid step constants ('diag' and the jump solver's 'indep'
+  // noise, no cross drift): every coefficient constant or every one
+  // proportional to the state.
+  k.uform = UFORM_GENERIC;
+  k.additive = 1;
+  for (int i = 0; i < m.dim; ++i)
+    if (m.j1[i] != 0.0) k.additive = 0;
+  for (int i = 0; i < SDEMC_MAX_DIM; ++i) {
+    k.uA[i] = k.uC[i] = 0.0f;
+    for (int q = 0; q < SDEMC_MAX_DIM; ++q) k.uW[i][q] = 0.0f;
+  }
+  if (m.noise == SDEMC_NOISE_DIAG || m.noise == SDEMC_NOISE_INDEP_COMMON) {
+    bool cst = true, prop = true;
+    for (int i = 0; i < m.dim; ++i) {
+      if (m.ax[i] != 0.0) cst = prop = false;
+      if (m.a1[i] != 0.0 || m.b1[i] != 0.0 || m.c1[i] != 0.0) cst = false;
+      if (m.a0[i] != 0.0 || m.b0[i] != 0.0 || m.c0[i] != 0.0) prop = false;
+    }
+    if (cst || prop) {
+      k.uform = cst ? UFORM_CONST : UFORM_PROP;
+      const double h = (double)k.h, sq = sqrt(h);
+      for (int i = 0; i < m.dim; ++i) {
+        const double a = cst ? m.a0[i] : m.a1[i];
+        const double b = cst ? m.b0[i] : m.b1[i];
+        const double c = cst ? m.c0[i] : m.c1[i];
+        k.uA[i] = (float)(a * h);
+        k.uC[i] = m.noise == SDEMC_NOISE_INDEP_COMMON ? (float)(c * sq) : 0.0f;
+        for (int q = 0; q <= i; ++q) k.uW[i][q] = (float)(b * m.chol[i * SDEMC_MAX_DIM + q] * sq);
+      }
+    }
   }
 }
 
@@ -564,6 +606,46 @@ __device__ __forceinline__ void correlate(const KModel& f, const float* z, float
 #pragma unroll
     for (int i = 0; i < D; ++i) dB[i] = 0.0f;
   }
+}
+
+// One Euler step of a uniform grid of step h = j.h from the step's K standard
+// normals z (the diffusion part; the caller adds the jumps).  Jobs with folded
+// constants (KJob::uform) take 3-5 operations per component instead of the
+// generic correlate + euler: x_i + uA_i + sum_k uW_ik z_k + uC_i z_D, or
+// x_i + x_i (uA_i + sum_k uW_ik z_k + uC_i z_D) -- the same Euler step
+// (schemes.py:5-10) with the products of the constant factors taken once on
+// the host.  Every kernel that steps a uniform grid from Philox normals
+// calls this, so their paths stay bit-identical.
+template <int D, int FAM>
+__device__ __forceinline__ void uniform_step(const KJob& j, float (&x)[D], const float* z) {
+  const KModel& f = j.m;
+  if constexpr (FAM == FAM_MERTON || FAM == FAM_LEVY) {
+    if (j.uform == UFORM_CONST) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        float t = x[i] + j.uA[i];
+        if (FAM == FAM_LEVY) t = __fmaf_rn(j.uC[i], z[D], t);
+#pragma unroll
+        for (int q = 0; q <= i; ++q) t = __fmaf_rn(j.uW[i][q], z[q], t);
+        x[i] = t;
+      }
+      return;
+    }
+    if (j.uform == UFORM_PROP) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        float g = __fmaf_rn(j.uW[i][0], z[0], j.uA[i]);
+#pragma unroll
+        for (int q = 1; q <= i; ++q) g = __fmaf_rn(j.uW[i][q], z[q], g);
+        if (FAM == FAM_LEVY) g = __fmaf_rn(j.uC[i], z[D], g);
+        x[i] = __fmaf_rn(x[i], g, x[i]);
+      }
+      return;
+    }
+  }
+  float dW[D], dB[D];
+  correlate<D, FAM>(f, z, j.sqh, dW, dB);
+  euler<D, FAM>(f, x, j.h, dW, dB);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,6 +861,29 @@ __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0,
     }
     const uint32_t e = have ? (uint32_t)(cnt - 1) : 0u;  // further marks, < 16
     uint32_t xb = POIS_NOX;
+    if (j.additive) {
+      // constant jump coefficients: the step adds j0 times the sum of its
+      // marks, so the item's lane draws the further marks itself and hands
+      // over one total (count 1)
+      if (__any(e != 0u)) {
+        const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)POIS_WIN));
+        const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)POIS_WIN));
+        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)POIS_WIN;
+        U4 mw = {0, 0, 0, 0};
+        for (uint32_t q = 0u; q < e; ++q) {
+          if ((q & 3u) == 0u) mw = philox4x32_10(n, STREAM_MARK + 16u * (q >> 2), olo, ohi, k0, k1);
+          const uint32_t sel = q & 3u;
+          const uint32_t u = sel == 0 ? mw.x : (sel == 1 ? mw.y : (sel == 2 ? mw.z : mw.w));
+          J += levy_icdf(j.m, u01_open(u));
+        }
+      }
+      if (have) {
+        L.v[ii] = __float_as_uint(J);
+        L.cnt[ii] = (uint8_t)1;
+        L.xb[ii] = (uint8_t)POIS_NOX;
+      }
+      continue;
+    }
     if (__any(e != 0u)) {
       uint32_t etot;
       const uint32_t xo = xused + wave_prefix<4>(e, etot);
@@ -957,7 +1062,6 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   constexpr int K = NK<D, FAM>::K;
   const KModel& f = j.m;
   const int32_t N = j.steps;
-  const float h = j.h;
   const float sq = j.sqh;
   const uint32_t plo = (uint32_t)c.path, phi = (uint32_t)(c.path >> 32);
   constexpr bool pois = POIS;
@@ -1047,14 +1151,18 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
       const int32_t n = n0 + s;
       if (n < N) {
         float dW[D], dB[D], xo[D];
-        correlate<D, FAM>(f, &z[s * K], sq, dW, dB);
+        // the increments themselves only for the coarse path and stored normals
+        if (COUPLED || (STORE && st.normals)) correlate<D, FAM>(f, &z[s * K], sq, dW, dB);
 #pragma unroll
         for (int i = 0; i < D; ++i) xo[i] = x[i];
-        euler<D, FAM>(f, x, h, dW, dB);
+        uniform_step<D, FAM>(j, x, &z[s * K]);
         float jsum = 0.0f;
         if (n == qk[0]) {
           do {
-            if (f.exact) {
+            if (j.additive) {  // fma(j1, y, j0) = j0 exactly: add_jump's bits, one operation
+#pragma unroll
+              for (int i = 0; i < D; ++i) x[i] = __fmaf_rn(f.j0[i], qJ[0], x[i]);
+            } else if (f.exact) {
               float y[D];
 #pragma unroll
               for (int i = 0; i < D; ++i) y[i] = x[i];
@@ -1072,7 +1180,15 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
             qk[JQ - 1] = -1;
           } while (n == qk[0]);
         }
-        if (pois && ((jm >> s) & 1u)) {
+        if (pois && j.additive && ((jm >> s) & 1u)) {
+          // constant jump coefficients: one total per jumping step
+          const uint32_t it = first + (uint32_t)__builtin_popcount(jm & ((1u << s) - 1u));
+          const float J = __uint_as_float(pl->v[it]);
+#pragma unroll
+          for (int i = 0; i < D; ++i) x[i] = __fmaf_rn(f.j0[i], J, x[i]);
+          jsum += J;
+          if constexpr (COUPLED) acc.jump(f, J);
+        } else if (pois && ((jm >> s) & 1u)) {
           const uint32_t it = first + (uint32_t)__builtin_popcount(jm & ((1u << s) - 1u));
           const PoisLds& L = *pl;
           const int cnt = L.cnt[it];
