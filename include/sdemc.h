@@ -230,6 +230,21 @@ int sdemc_simulate(const sdemc_job* job, double* stats_dev, const sdemc_path_out
                    void* workspace_dev, size_t workspace_bytes, int waves_per_block,
                    void* stream);
 
+/* The same plain path with no workspace argument and no allocation of any
+ * kind: the chunk partials live in a fixed device array of the library
+ * (SDEMC_NOALLOC_SCRATCH_BYTES, one per GPU, held until the process ends),
+ * so callers that cannot allocate -- code under HIP graph capture, serving
+ * loops that forbid allocation -- can price.  Jobs needing more workspace
+ * than the array holds (sdemc_workspace_bytes(n_paths) >
+ * SDEMC_NOALLOC_SCRATCH_BYTES, i.e. above SDEMC_NOALLOC_MAX_PATHS paths) get
+ * SDEMC_ERR_WORKSPACE.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other host threads) race.  No
+ * counterpart in the reference; its result equals sdemc_simulate's. */
+#define SDEMC_NOALLOC_MAX_PATHS (1ull << 24)
+#define SDEMC_NOALLOC_SCRATCH_BYTES (5u << 20) /* >= 20 B x 2^18 chunks, 256 B aligned */
+int sdemc_simulate_noalloc(const sdemc_job* job, double* stats_dev, const sdemc_path_out* out,
+                           int waves_per_block, void* stream);
+
 /* Plain path fed with host-generated increments, jump counts/times and marks
  * instead of in-kernel draws (parity mode; the HBM-bound variant). */
 int sdemc_simulate_fixed(const sdemc_job* job, const sdemc_fixed_in* in, double* stats_dev,

@@ -130,6 +130,9 @@ _SIGS = {
     "sdemc_simulate": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_void_p,
                                       ctypes.POINTER(PathOut), ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.c_int, ctypes.c_void_p]),
+    "sdemc_simulate_noalloc": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.c_void_p,
+                                              ctypes.POINTER(PathOut), ctypes.c_int,
+                                              ctypes.c_void_p]),
     "sdemc_simulate_fixed": (ctypes.c_int, [ctypes.POINTER(Job), ctypes.POINTER(FixedIn),
                                             ctypes.c_void_p, ctypes.POINTER(PathOut),
                                             ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

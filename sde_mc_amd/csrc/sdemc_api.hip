@@ -4,6 +4,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "sdemc_kernels.h"
 #include "cv_mlp.h"
 
@@ -18,6 +20,33 @@ static int hip_status(hipError_t e) {
 }
 
 static bool finite(double v) { return isfinite(v) != 0; }
+
+// Chunk-partial workspace of sdemc_simulate_noalloc: a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (on the host the name is only a
+// stand-in); it uses the device address hipGetSymbolAddress returns, looked
+// up once per device on its first call and kept in scratch_dev[].
+static __device__ __align__(256) unsigned char scratch[SDEMC_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static void* scratch_dev[kMaxDevices];
+static std::mutex scratch_mu;
+
+static int scratch_address(void** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e);
+  if (dev < 0 || dev >= kMaxDevices) return SDEMC_ERR_DEVICE;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_status(e);
+    if (!p) return SDEMC_ERR_DEVICE;
+    scratch_dev[dev] = p;
+  }
+  *ptr = scratch_dev[dev];
+  return SDEMC_OK;
+}
 
 // Host-side checks of one job; everything the kernels assume is enforced
 // here (dimensions, grid sizes, jump law constants, payoff kind).
@@ -285,6 +314,20 @@ int sdemc_simulate(const sdemc_job* job, double* stats_dev, const sdemc_path_out
     if (st != SDEMC_OK) return st;
   }
   return finalize_plain(*job, a.partials, a.cflags, stats_dev, a.stream);
+}
+
+int sdemc_simulate_noalloc(const sdemc_job* job, double* stats_dev, const sdemc_path_out* out,
+                           int waves_per_block, void* stream) {
+  int st = validate_job(job);
+  if (st != SDEMC_OK) return st;
+  const size_t need = workspace_bytes(job->n_paths);
+  if (need > (size_t)SDEMC_NOALLOC_SCRATCH_BYTES) return SDEMC_ERR_WORKSPACE;
+  void* ws = nullptr;
+  st = scratch_address(&ws);
+  if (st != SDEMC_OK) return st;
+  // every chunk's partial and flag is written before the finalisation reads
+  // them, so nothing left by an earlier call needs clearing
+  return sdemc_simulate(job, stats_dev, out, ws, need, waves_per_block, stream);
 }
 
 size_t sdemc_cv_mlp_words(int32_t hidden_layers) {

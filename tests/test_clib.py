@@ -178,3 +178,14 @@ def test_batch_check_rejects_misplanned_tables(name, mut):
     assert plan.hdr[0, 1] >= 2
     s, h, g, pc = mut(plan.segs.copy(), plan.hdr.copy(), plan.groups, chunks)
     assert _check(prepared, mj, s, h, g, pc) == L.ERR_ARG, name
+
+
+def test_noalloc_entry_point_refuses_before_touching_the_device():
+    # sizes beyond the library's array and invalid jobs are refused on the host
+    lib = L.load()
+    job = J.job_for(J.merton_sde(), 1.0, 10, (1 << 24) + 1)
+    assert lib.sdemc_simulate_noalloc(ctypes.byref(job), ctypes.c_void_p(8), None, 4, None) == \
+        L.ERR_WORKSPACE
+    job.steps = 0
+    assert lib.sdemc_simulate_noalloc(ctypes.byref(job), ctypes.c_void_p(8), None, 4, None) == \
+        L.ERR_ARG
