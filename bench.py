@@ -261,19 +261,29 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
                           ctypes.byref(ws))
     per_iter = {0: 32, 1: 64, 2: 64, 3: 96}   # VALU instructions per loop iteration
     names = {0: "v_fma_f32", 1: "v_xor_b32 + v_add_u32", 2: "v_mad_u64_u32 + v_xor_b32",
-             3: "v_exp_f32 + 2 v_mul_f32"}
+             3: "v_exp_f32 + 2 v_mul_f32", 4: "philox4x32_10 block",
+             5: "philox4x32_10 block + 2 box_muller pairs"}
     rec = {"clock_mhz": clock_khz.value / 1e3, "cus": cus,
            "probe": "sdemc_valu_probe_dev, %d workgroups of 4 wavefronts, %d iterations" % (blocks, iters)}
-    for kind in (0, 1, 2, 3):
+    waves = blocks * 4
+    for kind in (0, 1, 2, 3, 4, 5):
+        it = iters if kind < 4 else iters // 4
+
         def run():
-            L.check(lib.sdemc_valu_probe_dev(ctypes.c_void_p(out.data_ptr()), blocks, iters, kind,
+            L.check(lib.sdemc_valu_probe_dev(ctypes.c_void_p(out.data_ptr()), blocks, it, kind,
                                              E.stream_ptr()), 'sdemc_valu_probe_dev')
         run()
         torch.cuda.synchronize()
         ms = min(_time_events(run, 1) for _ in range(reps))
-        rate = blocks * 4 * per_iter[kind] * iters / (ms * 1e-3)
-        rec[names[kind]] = {"wave_instr_per_s": rate, "kernel_ms": ms,
-                            "wave_instr_per_cu_cycle": rate / (cus * clock_khz.value * 1e3)}
+        cycles = ms * 1e-3 * clock_khz.value * 1e3
+        if kind < 4:
+            rate = waves * per_iter[kind] * it / (ms * 1e-3)
+            rec[names[kind]] = {"wave_instr_per_s": rate, "kernel_ms": ms,
+                                "wave_instr_per_cu_cycle": rate / (cus * clock_khz.value * 1e3)}
+        else:
+            # SIMD cycles one wavefront's iteration occupies (4 SIMDs per CU)
+            rec[names[kind]] = {"kernel_ms": ms,
+                                "simd_cycles_per_wave_iter": cycles * cus * 4 / (waves * it)}
     rec["wave_instr_per_s"] = rec[names[0]]["wave_instr_per_s"]
     return rec
 

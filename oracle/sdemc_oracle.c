@@ -65,7 +65,8 @@ static void philox_block(uint32_t blk, uint32_t stream, uint64_t path, uint64_t 
 }
 
 static double u_open(uint32_t w) { return (double)((w >> 8) | 1u) * 0x1p-24; }
-static double u_half(uint32_t w) { return (double)(w >> 8) * 0x1p-24; }
+/* Box-Muller angle in revolutions: the top 23 bits of the word */
+static double u_angle(uint32_t w) { return (double)(w >> 9) * 0x1p-23; }
 
 /* ln u for u = (w | 1) 2^-32, fp32: u = m 2^e, m in [sqrt(1/2), sqrt(2)),
  * ln m = 2 atanh(s) with s = (m-1)/(m+1) by the odd series to s^9.  Every
@@ -119,14 +120,16 @@ ORACLE_API int oracle_poisson_thresholds(double mu, uint32_t* T16, double* inv) 
   return poisson_thresholds(mu, T16, inv);
 }
 
-/* normal i of a path: word i%4 of Brownian block i/4, Box-Muller pairs (0,1),(2,3) */
+/* normal i of a path: word i%4 of Brownian block i/4, Box-Muller pairs (0,1),(2,3):
+ * radius from the first word's uniform ((w >> 8) | 1) 2^-24, angle 2 pi (w >> 9) 2^-23
+ * from the second */
 ORACLE_API double oracle_normal(uint64_t seed, uint64_t path, uint32_t i) {
   uint32_t w[4];
   philox_block(i >> 2, 0, path, seed, w);
   uint32_t s = i & 3u;
   uint32_t a = w[(s >> 1) * 2], b = w[(s >> 1) * 2 + 1];
   double r = sqrt(-2.0 * log(u_open(a)));
-  double th = 2.0 * M_PI * u_half(b);
+  double th = 2.0 * M_PI * u_angle(b);
   return (s & 1u) ? r * sin(th) : r * cos(th);
 }
 
@@ -193,7 +196,7 @@ static double mark_from_words(const model_t* m, uint32_t m1, uint32_t m2) {
     volatile float y = base + 0x1.555556p-26f;
     return levy_icdf(m, y);
   }
-  double z = sqrt(-2.0 * log(u_open(m1))) * cos(2.0 * M_PI * u_half(m2));
+  double z = sqrt(-2.0 * log(u_open(m1))) * cos(2.0 * M_PI * u_angle(m2));
   return exp(md->jp[0] + md->jp[1] * z) - 1.0;
 }
 

@@ -90,19 +90,28 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, u
   return U4{c0, c1, c2, c3};
 }
 
-// Uniform in (0,1): odd multiples of 2^-24, exact in fp32 (and in fp64).
+// 1 + (w >> 9) 2^-23 in [1, 2), assembled as float bits by one
+// v_alignbit_b32: ({0x7F, w} >> 9) = 0x3F800000 | (w >> 9).
+__device__ __forceinline__ float one_plus_u23(uint32_t w) {
+  return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w, 9u));
+}
+// Uniform in (0,1): odd multiples of 2^-24, exact in fp32 (and in fp64):
+// ((w >> 8) | 1) 2^-24 = (1 + (w >> 9) 2^-23) - (1 - 2^-24), the
+// subtraction exact (Sterbenz).
 __device__ __forceinline__ float u01_open(uint32_t w) {
-  return (float)((w >> 8) | 1u) * 0x1p-24f;
+  return one_plus_u23(w) - 0x1.fffffep-1f;
 }
 // Uniform in [0,1): multiples of 2^-24, exact in fp32.
 __device__ __forceinline__ float u01_half(uint32_t w) { return (float)(w >> 8) * 0x1p-24f; }
 
 // Box-Muller on the hardware transcendental units: v_log_f32 is log2,
-// v_sin_f32 / v_cos_f32 take the angle in revolutions, so 2*pi*a needs no
-// range reduction.  z0 = r cos(2 pi a), z1 = r sin(2 pi a), r = sqrt(-2 ln u).
+// v_sin_f32 / v_cos_f32 take the angle in revolutions and are periodic in
+// it, so the angle a = (w1 >> 9) 2^-23 is passed as 1 + a, assembled in one
+// operation, and needs no range reduction.  z0 = r cos(2 pi a),
+// z1 = r sin(2 pi a), r = sqrt(-2 ln u).
 __device__ __forceinline__ void box_muller(uint32_t w0, uint32_t w1, float& z0, float& z1) {
   const float u = u01_open(w0);
-  const float a = u01_half(w1);
+  const float a = one_plus_u23(w1);
   const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u));
   z0 = r * __builtin_amdgcn_cosf(a);
   z1 = r * __builtin_amdgcn_sinf(a);
@@ -441,7 +450,7 @@ __device__ __forceinline__ float jump_mark(const KModel& f, uint32_t m1, uint32_
     return levy_icdf(f, y);
   }
   const float u = u01_open(m1);
-  const float a = u01_half(m2);
+  const float a = one_plus_u23(m2);  // the angle (m2 >> 9) 2^-23, as in box_muller
   const float z = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u)) *
                   __builtin_amdgcn_cosf(a);
   return fast_exp(__fmaf_rn(f.js, z, f.jm)) - 1.0f;
