@@ -173,18 +173,24 @@ __global__ void k_normals(uint32_t k0, uint32_t k1, uint64_t path_offset, int64_
 //   KIND 4  one Philox4x32-10 block per iteration (the kernels' generator)
 //   KIND 5  a Philox block and two Box-Muller pairs: the 4 normals of one
 //           block as the path kernels draw them
+//   KIND 6  two independent Philox blocks per iteration (the same work as
+//           two iterations of KIND 4 with twice the instruction-level
+//           parallelism)
 // KIND 0 measures the ceiling the path kernels' VALU instruction counts are
 // priced at; the others the costs of Philox (1, 2, 4) and Box-Muller (3, 5).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int32_t iters,
                                                     float a, float b) {
-  if constexpr (KIND == 4 || KIND == 5) {
+  if constexpr (KIND == 4 || KIND == 5 || KIND == 6) {
     const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t acc = 0u;
     float zacc = 0.0f;
     for (int32_t i = 0; i < iters; ++i) {
       const U4 w = philox4x32_10((uint32_t)i, STREAM_BM, path, 0u, __float_as_uint(a), 7u);
-      if constexpr (KIND == 4) {
+      if constexpr (KIND == 6) {
+        const U4 v = philox4x32_10((uint32_t)i, STREAM_JUMP, path, 0u, __float_as_uint(a), 7u);
+        acc ^= (w.x ^ w.y ^ w.z ^ w.w) + (v.x ^ v.y ^ v.z ^ v.w);
+      } else if constexpr (KIND == 4) {
         acc ^= w.x ^ w.y ^ w.z ^ w.w;
       } else {
         float z0, z1, z2, z3;
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int
       }
     }
     out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] =
-        KIND == 4 ? __uint_as_float(acc) : zacc;
+        KIND == 5 ? zacc : __uint_as_float(acc);
   } else if constexpr (KIND == 0 || KIND == 3) {
     float x[8];
     const float s = (float)threadIdx.x * 1e-3f;
@@ -473,7 +479,7 @@ int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint
 
 int sdemc_valu_probe_dev(float* out_dev, int32_t blocks, int32_t iters, int32_t kind,
                          void* stream) {
-  if (!out_dev || blocks < 1 || iters < 1 || kind < 0 || kind > 5) return SDEMC_ERR_ARG;
+  if (!out_dev || blocks < 1 || iters < 1 || kind < 0 || kind > 6) return SDEMC_ERR_ARG;
   const dim3 g((unsigned)blocks), b(256);
   const hipStream_t s = (hipStream_t)stream;
   switch (kind) {
@@ -482,6 +488,7 @@ int sdemc_valu_probe_dev(float* out_dev, int32_t blocks, int32_t iters, int32_t 
     case 2: hipLaunchKernelGGL(k_valu_probe<2>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
     case 4: hipLaunchKernelGGL(k_valu_probe<4>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
     case 5: hipLaunchKernelGGL(k_valu_probe<5>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
+    case 6: hipLaunchKernelGGL(k_valu_probe<6>, g, b, 0, s, out_dev, iters, 0.99999f, 1e-6f); break;
     default: hipLaunchKernelGGL(k_valu_probe<3>, g, b, 0, s, out_dev, iters, 0.001f, 0.999f); break;
   }
   return hip_status(hipGetLastError());
