@@ -262,7 +262,7 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
     per_iter = {0: 32, 1: 64, 2: 64, 3: 96}   # VALU instructions per loop iteration
     names = {0: "v_fma_f32", 1: "v_xor_b32 + v_add_u32", 2: "v_mad_u64_u32 + v_xor_b32",
              3: "v_exp_f32 + 2 v_mul_f32", 4: "philox4x32_10 block",
-             5: "philox4x32_10 block + 2 box_muller pairs",
+             5: "philox4x32_10 block + 3 box_muller pairs (6 normals)",
              6: "2 independent philox4x32_10 blocks"}
     rec = {"clock_mhz": clock_khz.value / 1e3, "cus": cus,
            "probe": "sdemc_valu_probe_dev, %d workgroups of 4 wavefronts, %d iterations" % (blocks, iters)}

@@ -459,7 +459,7 @@ int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint
  * (v_fma_f32; a launch issues blocks * 4 * 32 * iters wavefront FMAs);
  * 1: 32-bit integer xor + add; 2: 32 x 32 -> 64-bit multiply; 3: fp32
  * exp2 and two multiplies; 4: one Philox4x32-10 block per iteration (not 32
- * operations); 5: a Philox block and the two Box-Muller pairs of its four
+ * operations); 5: a Philox block and the three Box-Muller pairs of its six
  * normals per iteration; 6: two independent Philox blocks per iteration. */
 int sdemc_valu_probe_dev(float* out_dev, int32_t blocks, int32_t iters, int32_t kind,
                          void* stream);

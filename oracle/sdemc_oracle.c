@@ -120,16 +120,17 @@ ORACLE_API int oracle_poisson_thresholds(double mu, uint32_t* T16, double* inv) 
   return poisson_thresholds(mu, T16, inv);
 }
 
-/* normal i of a path: word i%4 of Brownian block i/4, Box-Muller pairs (0,1),(2,3):
- * radius from the first word's uniform ((w >> 8) | 1) 2^-24, angle 2 pi (w >> 9) 2^-23
- * from the second */
+/* normal i of a path: normal i%6 of Brownian block i/6, six per block from three
+ * Box-Muller pairs: pair p takes its radius uniform ((w_p >> 8) | 1) 2^-24 from
+ * word p and its angle a = ((w_p mod 2^9) 2^10 + bits [10p, 10p + 10) of w_3) 2^-19;
+ * z_2p = r cos 2 pi a, z_2p+1 = r sin 2 pi a */
 ORACLE_API double oracle_normal(uint64_t seed, uint64_t path, uint32_t i) {
   uint32_t w[4];
-  philox_block(i >> 2, 0, path, seed, w);
-  uint32_t s = i & 3u;
-  uint32_t a = w[(s >> 1) * 2], b = w[(s >> 1) * 2 + 1];
-  double r = sqrt(-2.0 * log(u_open(a)));
-  double th = 2.0 * M_PI * u_angle(b);
+  philox_block(i / 6u, 0, path, seed, w);
+  uint32_t s = i % 6u, p = s >> 1;
+  uint32_t ang = ((w[p] & 0x1FFu) << 10) | ((w[3] >> (10u * p)) & 0x3FFu);
+  double r = sqrt(-2.0 * log(u_open(w[p])));
+  double th = 2.0 * M_PI * (double)ang * 0x1p-19;
   return (s & 1u) ? r * sin(th) : r * cos(th);
 }
 

@@ -171,7 +171,7 @@ __global__ void k_normals(uint32_t k0, uint32_t k1, uint64_t path_offset, int64_
 //   KIND 2  32 x 32 -> 64-bit multiply   (hi, lo) of x * a, x = hi ^ lo
 //   KIND 3  fp32 transcendental + FMA    x = exp2(x a) b
 //   KIND 4  one Philox4x32-10 block per iteration (the kernels' generator)
-//   KIND 5  a Philox block and two Box-Muller pairs: the 4 normals of one
+//   KIND 5  a Philox block and three Box-Muller pairs: the 6 normals of one
 //           block as the path kernels draw them
 //   KIND 6  two independent Philox blocks per iteration (the same work as
 //           two iterations of KIND 4 with twice the instruction-level
@@ -193,10 +193,9 @@ __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int
       } else if constexpr (KIND == 4) {
         acc ^= w.x ^ w.y ^ w.z ^ w.w;
       } else {
-        float z0, z1, z2, z3;
-        box_muller(w.x, w.y, z0, z1);
-        box_muller(w.z, w.w, z2, z3);
-        zacc += (z0 + z1) + (z2 + z3);
+        float z[NORMALS_PER_BLOCK];
+        normals6(w, z);
+        zacc += ((z[0] + z[1]) + (z[2] + z[3])) + (z[4] + z[5]);
       }
     }
     out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] =

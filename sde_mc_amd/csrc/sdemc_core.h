@@ -106,16 +106,41 @@ __device__ __forceinline__ float u01_half(uint32_t w) { return (float)(w >> 8) *
 
 // Box-Muller on the hardware transcendental units: v_log_f32 is log2,
 // v_sin_f32 / v_cos_f32 take the angle in revolutions and are periodic in
-// it, so the angle a = (w1 >> 9) 2^-23 is passed as 1 + a, assembled in one
-// operation, and needs no range reduction.  z0 = r cos(2 pi a),
-// z1 = r sin(2 pi a), r = sqrt(-2 ln u).
-__device__ __forceinline__ void box_muller(uint32_t w0, uint32_t w1, float& z0, float& z1) {
-  const float u = u01_open(w0);
-  const float a = one_plus_u23(w1);
+// it, so an angle a in [0, 1) is passed as the float 1 + a and needs no range
+// reduction.  z0 = r cos(2 pi a), z1 = r sin(2 pi a), r = sqrt(-2 ln u).
+__device__ __forceinline__ void box_muller_ua(float u, float one_plus_a, float& z0, float& z1) {
   const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u));
-  z0 = r * __builtin_amdgcn_cosf(a);
-  z1 = r * __builtin_amdgcn_sinf(a);
+  z0 = r * __builtin_amdgcn_cosf(one_plus_a);
+  z1 = r * __builtin_amdgcn_sinf(one_plus_a);
 }
+
+// Six standard normals from one Philox block (w0, w1, w2, w3), three
+// Box-Muller pairs: pair p takes its radius uniform from word p,
+// u = ((w_p >> 8) | 1) 2^-24, and a 19-bit angle a = ((w_p mod 2^9) 2^10 +
+// bits [10p, 10p + 10) of w3) 2^-19 from the 9 low bits of word p, which the
+// radius does not use, and 10 bits of word 3 (126 of the block's 128 bits).
+// z[2p] = r_p cos(2 pi a_p), z[2p + 1] = r_p sin(2 pi a_p).
+constexpr int NORMALS_PER_BLOCK = 6;
+__device__ __forceinline__ void normals6(const U4& w, float* z) {
+  const uint32_t wp[3] = {w.x, w.y, w.z};
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    // 1 + a as float bits: 0x3F800000 | (w_p mod 2^9) << 14 | (w3 field) << 4
+    const uint32_t lo = (((w.w >> (10 * p)) & 0x3FFu) << 4) | 0x3F800000u;
+    const uint32_t bits = ((wp[p] << 14) & 0x007FC000u) | lo;
+    box_muller_ua(u01_open(wp[p]), __uint_as_float(bits), z[2 * p], z[2 * p + 1]);
+  }
+}
+
+// Normals of block b of a path's Brownian stream.
+__device__ __forceinline__ void normal_block(uint32_t b, uint32_t plo, uint32_t phi, uint32_t k0,
+                                             uint32_t k1, float* z) {
+  normals6(philox4x32_10(b, STREAM_BM, plo, phi, k0, k1), z);
+}
+
+// Steps (or adapted-grid rows) whose K normals each are drawn together as
+// whole blocks: 4 when 4K normals fill whole blocks (K = 3), else 6.
+__host__ __device__ constexpr int group_steps(int K) { return (4 * K) % NORMALS_PER_BLOCK == 0 ? 4 : 6; }
 
 __device__ __forceinline__ float fast_exp(float v) {
   return __builtin_amdgcn_exp2f(v * 1.44269504088896341f);
@@ -154,13 +179,13 @@ __device__ __forceinline__ float ln_unit(uint32_t w) {
   return lnm + et;
 }
 
-// Normal number i of one path (i = step * K + k): word (i mod 4) of Philox
-// block i / 4 on the Brownian stream, paired (0,1) and (2,3) by Box-Muller.
-// The last block is cached, so sequential reads cost one Philox per 4 normals.
+// Normal number i of one path (i = step * K + k): normal i mod 6 of Philox
+// block i / 6 on the Brownian stream (normals6).  The last block is cached,
+// so sequential reads cost one Philox block per 6 normals.
 struct NormalStream {
   uint32_t k0, k1, plo, phi;
   uint32_t blk;
-  float z0, z1, z2, z3;
+  float z[NORMALS_PER_BLOCK];
   __device__ __forceinline__ void init(uint32_t key0, uint32_t key1, uint64_t path) {
     k0 = key0;
     k1 = key1;
@@ -169,15 +194,16 @@ struct NormalStream {
     blk = 0xffffffffu;
   }
   __device__ __forceinline__ float get(uint32_t i) {
-    const uint32_t b = i >> 2;
+    const uint32_t b = i / (uint32_t)NORMALS_PER_BLOCK;
     if (b != blk) {
       blk = b;
-      const U4 w = philox4x32_10(b, STREAM_BM, plo, phi, k0, k1);
-      box_muller(w.x, w.y, z0, z1);
-      box_muller(w.z, w.w, z2, z3);
+      normal_block(b, plo, phi, k0, k1, z);
     }
-    const uint32_t s = i & 3u;
-    return s == 0 ? z0 : (s == 1 ? z1 : (s == 2 ? z2 : z3));
+    const uint32_t s = i - b * (uint32_t)NORMALS_PER_BLOCK;
+    float v = z[0];
+#pragma unroll
+    for (int q = 1; q < NORMALS_PER_BLOCK; ++q) v = s == (uint32_t)q ? z[q] : v;
+    return v;
   }
 };
 
@@ -450,7 +476,7 @@ __device__ __forceinline__ float jump_mark(const KModel& f, uint32_t m1, uint32_
     return levy_icdf(f, y);
   }
   const float u = u01_open(m1);
-  const float a = one_plus_u23(m2);  // the angle (m2 >> 9) 2^-23, as in box_muller
+  const float a = one_plus_u23(m2);  // the angle (m2 >> 9) 2^-23 in revolutions, plus 1
   const float z = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u)) *
                   __builtin_amdgcn_cosf(a);
   return fast_exp(__fmaf_rn(f.js, z, f.jm)) - 1.0f;
@@ -780,7 +806,7 @@ __device__ __forceinline__ float poisson_jumps(const KJob& j, uint32_t k0, uint3
 // (two 4-word count blocks) fill the passes better than four and halve the
 // fixed cost of the list per step.
 // ---------------------------------------------------------------------------
-constexpr int POIS_WIN = 8;         // steps per window
+constexpr int POIS_WIN = 8;         // steps per window at most (8 for 4-step groups, else 6)
 constexpr int POIS_XCAP = 128;      // marks 2, 3, ... held per window
 constexpr uint32_t POIS_NOX = 0xFFu;  // item's further marks drawn by its owner
 
@@ -830,10 +856,12 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
 // jmask: the lane's jumping steps, w: its count words.  Returns the lane's
 // first item; the item of step s is that plus the number of jumping steps
 // before s.  Must be called by every lane of the wavefront together.
+template <int WIN>
 __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0, uint32_t k1,
                                                      uint32_t plo, uint32_t phi, int32_t n0,
-                                                     const uint32_t (&w)[POIS_WIN],
+                                                     const uint32_t (&w)[WIN],
                                                      uint32_t jmask, PoisLds& L) {
+  static_assert(WIN <= POIS_WIN, "window larger than the item list");
   uint32_t total;
   const uint32_t first = wave_prefix<4>((uint32_t)__builtin_popcount(jmask), total);
   if (total == 0u) return first;
@@ -842,10 +870,10 @@ __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0,
   {
     uint32_t it = first;
 #pragma unroll
-    for (int s = 0; s < POIS_WIN; ++s) {
+    for (int s = 0; s < WIN; ++s) {
       if ((jmask >> s) & 1u) {
         L.v[it] = w[s];
-        L.own[it] = (uint16_t)(lane * (uint32_t)POIS_WIN + (uint32_t)s);
+        L.own[it] = (uint16_t)(lane * (uint32_t)WIN + (uint32_t)s);
         ++it;
       }
     }
@@ -875,9 +903,9 @@ __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0,
       // marks, so the item's lane draws the further marks itself and hands
       // over one total (count 1)
       if (__any(e != 0u)) {
-        const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)POIS_WIN));
-        const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)POIS_WIN));
-        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)POIS_WIN;
+        const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)WIN));
+        const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)WIN));
+        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)WIN;
         U4 mw = {0, 0, 0, 0};
         for (uint32_t q = 0u; q < e; ++q) {
           if ((q & 3u) == 0u) mw = philox4x32_10(n, STREAM_MARK + 16u * (q >> 2), olo, ohi, k0, k1);
@@ -897,11 +925,11 @@ __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0,
       uint32_t etot;
       const uint32_t xo = xused + wave_prefix<4>(e, etot);
       xused += etot;
-      const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)POIS_WIN));
-      const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)POIS_WIN));
+      const uint32_t olo = __shfl(plo, (int)(ow / (uint32_t)WIN));
+      const uint32_t ohi = __shfl(phi, (int)(ow / (uint32_t)WIN));
       if (e != 0u && xo + e <= (uint32_t)POIS_XCAP) {
         xb = xo;
-        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)POIS_WIN;
+        const uint32_t n = (uint32_t)n0 + ow % (uint32_t)WIN;
         U4 mw = {0, 0, 0, 0};
         for (uint32_t q = 0u; q < e; ++q) {
           // mark q + 2 of the step: word q mod 4 of mark-stream block q / 4
@@ -1053,11 +1081,11 @@ struct CoarseAcc {
 };
 
 // ===========================================================================
-// Uniform grid, single level: N steps of h = fp32(T/N).  The 4K normals of
-// four steps come from K Philox blocks generated together, so every index
-// into them is a compile-time constant.
+// Uniform grid, single level: N steps of h = fp32(T/N).  The G K normals of
+// a group of G steps (group_steps: 4 or 6) come from G K / 6 Philox blocks
+// generated together, so every index into them is a compile-time constant.
 // ===========================================================================
-// U: 4-step groups per iteration of the step loop (code shape only; the
+// U: G-step groups per iteration of the step loop (code shape only; the
 // draws and the arithmetic are the same for every U).
 // POIS: the job's jump mode (per-step Poisson counts or the arrival clock),
 // a template parameter so each form keeps only its own state in registers;
@@ -1069,6 +1097,8 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
                                       float (&x)[D], const StoreCtx& st,
                                       float* xc = nullptr) {
   constexpr int K = NK<D, FAM>::K;
+  constexpr int G = group_steps(K);         // steps per group (whole normal blocks)
+  constexpr int WIN = G == 4 ? 8 : 6;       // steps per Poisson window, a multiple of G
   const KModel& f = j.m;
   const int32_t N = j.steps;
   const float sq = j.sqh;
@@ -1085,11 +1115,11 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   // the order the per-step form applies them, by passes in which every lane
   // of the wavefront takes its next jump at once, so the draw code runs
   // about max-over-lanes times per window of clock_win steps (sized by the
-  // job's jump rate, rounded up to whole strides of 4U) instead of once per
+  // job's jump rate, rounded up to whole strides of G U) instead of once per
   // step in which any lane jumps; a step checks only the queue's head, and
   // jumps beyond the queue are taken inside the step loop by code compiled
   // into the windows where some lane needs it.  Per-step Poisson mode: the
-  // jumping steps of each window of POIS_WIN steps are drawn by the whole
+  // jumping steps of each window of WIN steps are drawn by the whole
   // wavefront from a list in LDS (pois_draw_window), and each step reads its
   // marks back.  Marks, order and arithmetic are those of the per-step form,
   // so results are unchanged.
@@ -1122,41 +1152,54 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     uint32_t jm = 0u;     // the lane's jumping steps of the group
     uint32_t first = 0u;  // its first item in the window's jump list
     if (pois) {
-      if ((n0 & (POIS_WIN - 1)) == 0) {
-        // a new window: the count words of its steps (4 per Philox block)
-        uint32_t w[POIS_WIN];
+      if ((n0 % WIN) == 0) {
+        // a new window: the count words of its steps (step n: word n mod 4
+        // of block n / 4 of the count stream)
+        uint32_t w[WIN];
         const uint32_t T0 = j.pois_T[0];
         wjm = 0u;
+        // blocks a window touches: two (8 steps from word 0, or 6 from word 0 or 2)
+        static_assert(WIN == 8 || WIN == 6, "count-word windows of 8 or 6 steps");
+        constexpr int NB = 2;
+        uint32_t cw[4 * NB];
 #pragma unroll
-        for (int g = 0; g < POIS_WIN / 4; ++g) {
+        for (int g = 0; g < NB; ++g) {
           U4 pw = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-          if (n0 + 4 * g < N)
+          const int32_t nb = ((n0 >> 2) + g) * 4;  // first step of the block
+          if (nb < N)
             pw = philox4x32_10((uint32_t)(n0 >> 2) + (uint32_t)g, STREAM_POIS, plo, phi, c.k0, c.k1);
-          w[4 * g + 0] = pw.x;
-          w[4 * g + 1] = pw.y;
-          w[4 * g + 2] = pw.z;
-          w[4 * g + 3] = pw.w;
+          cw[4 * g + 0] = pw.x;
+          cw[4 * g + 1] = pw.y;
+          cw[4 * g + 2] = pw.z;
+          cw[4 * g + 3] = pw.w;
+        }
+        if constexpr (WIN % 4 == 0) {
+#pragma unroll
+          for (int q = 0; q < WIN; ++q) w[q] = cw[q];
+        } else {
+          // a 6-step window starts at word 0 or 2 of its first block
+          const bool odd = (n0 & 3) != 0;
+#pragma unroll
+          for (int q = 0; q < WIN; ++q) w[q] = odd ? cw[q + 2] : cw[q];
         }
 #pragma unroll
-        for (int q = 0; q < POIS_WIN; ++q) wjm |= (w[q] < T0 ? 1u : 0u) << q;
+        for (int q = 0; q < WIN; ++q) wjm |= (w[q] < T0 ? 1u : 0u) << q;
         const int32_t inside = N - n0;
-        if (inside < POIS_WIN) wjm &= (1u << inside) - 1u;
-        wfirst = pois_draw_window(j, c.k0, c.k1, plo, phi, n0, w, wjm, *pl);
+        if (inside < WIN) wjm &= (1u << inside) - 1u;
+        wfirst = pois_draw_window<WIN>(j, c.k0, c.k1, plo, phi, n0, w, wjm, *pl);
       }
-      const uint32_t sh = (uint32_t)(n0 & (POIS_WIN - 1));
-      jm = (wjm >> sh) & 15u;
+      const uint32_t sh = (uint32_t)(n0 % WIN);
+      jm = (wjm >> sh) & ((1u << G) - 1u);
       first = wfirst + (uint32_t)__builtin_popcount(wjm & ((1u << sh) - 1u));
     }
-    float z[4 * K];
-    const uint32_t b0 = (uint32_t)(n0 >> 2) * (uint32_t)K;
+    // the G K normals of the group's steps: G K / 6 whole blocks
+    float z[G * K];
+    const uint32_t b0 = (uint32_t)n0 * (uint32_t)K / (uint32_t)NORMALS_PER_BLOCK;
 #pragma unroll
-    for (int b = 0; b < K; ++b) {
-      const U4 w = philox4x32_10(b0 + b, STREAM_BM, plo, phi, c.k0, c.k1);
-      box_muller(w.x, w.y, z[4 * b + 0], z[4 * b + 1]);
-      box_muller(w.z, w.w, z[4 * b + 2], z[4 * b + 3]);
-    }
+    for (int b = 0; b < G * K / NORMALS_PER_BLOCK; ++b)
+      normal_block(b0 + b, plo, phi, c.k0, c.k1, &z[NORMALS_PER_BLOCK * b]);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < G; ++s) {
       const int32_t n = n0 + s;
       if (n < N) {
         float dW[D], dB[D], xo[D];
@@ -1271,10 +1314,10 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     }
   };
   // arrival-clock fill windows: a whole number of step-loop strides
-  const int32_t win = ((j.clock_win + 4 * U - 1) / (4 * U)) * (4 * U);
+  const int32_t win = ((j.clock_win + G * U - 1) / (G * U)) * (G * U);
   int32_t wend = 0;
   bool spill = false;
-  for (int32_t n0 = 0; n0 < N; n0 += 4 * U) {
+  for (int32_t n0 = 0; n0 < N; n0 += G * U) {
     if (clock && n0 >= wend) {
       wend = n0 + win;
 #pragma unroll
@@ -1295,12 +1338,12 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
       group(n0, std::true_type{});
 #pragma unroll
       for (int u = 1; u < U; ++u)
-        if (n0 + 4 * u < N) group(n0 + 4 * u, std::true_type{});
+        if (n0 + G * u < N) group(n0 + G * u, std::true_type{});
     } else {
       group(n0, std::false_type{});
 #pragma unroll
       for (int u = 1; u < U; ++u)
-        if (n0 + 4 * u < N) group(n0 + 4 * u, std::false_type{});
+        if (n0 + G * u < N) group(n0 + G * u, std::false_type{});
     }
   }
 }
@@ -1581,17 +1624,15 @@ struct AdaptRngSrc {
     constexpr int K = NK<D, FAM>::K;
     return ns.get(row * K + k);
   }
-  // normals of rows row0 .. row0 + 3 (row0 a multiple of 4): K whole blocks,
-  // zq[s K + k] = z(row0 + s, k)
-  __device__ __forceinline__ void rows4(uint32_t row0, bool, float (&zq)[4 * NK<D, FAM>::K]) {
+  // normals of rows row0 .. row0 + G - 1 (G = group_steps(K), row0 a
+  // multiple of G): G K / 6 whole blocks, zq[s K + k] = z(row0 + s, k)
+  static constexpr int G = group_steps(NK<D, FAM>::K);
+  __device__ __forceinline__ void rows(uint32_t row0, bool, float (&zq)[G * NK<D, FAM>::K]) {
     constexpr int K = NK<D, FAM>::K;
-    const uint32_t b0 = (row0 >> 2) * (uint32_t)K;
+    const uint32_t b0 = row0 * (uint32_t)K / (uint32_t)NORMALS_PER_BLOCK;
 #pragma unroll
-    for (int b = 0; b < K; ++b) {
-      const U4 w = philox4x32_10(b0 + b, STREAM_BM, plo, phi, k0, k1);
-      box_muller(w.x, w.y, zq[4 * b + 0], zq[4 * b + 1]);
-      box_muller(w.z, w.w, zq[4 * b + 2], zq[4 * b + 3]);
-    }
+    for (int b = 0; b < G * K / NORMALS_PER_BLOCK; ++b)
+      normal_block(b0 + b, plo, phi, k0, k1, &zq[NORMALS_PER_BLOCK * b]);
   }
   __device__ __forceinline__ float next_time() const { return tau1; }
   __device__ __forceinline__ float mark(const KModel&) const { return J1; }
@@ -1635,12 +1676,13 @@ struct AdaptFixedSrc {
     if ((int32_t)row >= in->n_rows) return 0.0f;
     return in->dw_dev[((int64_t)row * K + k) * P + p];
   }
-  // rows row0 .. row0 + 3 of a live lane (a lane past the last path, or one
-  // whose path has ended, reads nothing)
-  __device__ __forceinline__ void rows4(uint32_t row0, bool live, float (&zq)[4 * NK<D, FAM>::K]) {
+  // rows row0 .. row0 + G - 1 of a live lane (a lane past the last path, or
+  // one whose path has ended, reads nothing)
+  static constexpr int G = group_steps(NK<D, FAM>::K);
+  __device__ __forceinline__ void rows(uint32_t row0, bool live, float (&zq)[G * NK<D, FAM>::K]) {
     constexpr int K = NK<D, FAM>::K;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < G; ++q)
 #pragma unroll
       for (int k = 0; k < K; ++k) zq[q * K + k] = live ? z(row0 + (uint32_t)q, k) : 0.0f;
   }
@@ -1680,18 +1722,21 @@ __device__ int32_t path_adapted(const KJob& j, SRC& src, bool lane_valid, int64_
     if (st.times) st.times[p] = 0.0f;
   }
   // A live lane takes one step per iteration, so at the start of iteration i
-  // every live lane is at row i: the normals of rows i .. i + 3 are drawn for
-  // the wavefront together every 4 iterations, in the stream's layout (row r,
-  // normal k is index r K + k).  The second jump slots are refilled every
-  // clock_win iterations (a power of two from 4: about half a jump per path
-  // between refills), a jump beyond them being drawn in the step.
+  // every live lane is at row i: the normals of rows i .. i + G - 1 are drawn
+  // for the wavefront together every G iterations (G = group_steps(K), whole
+  // blocks), in the stream's layout (row r, normal k is index r K + k).  The
+  // second jump slots are refilled in the first group at or after every
+  // multiple of clock_win iterations (a power of two from 4: about half a
+  // jump per path between refills), a jump beyond them being drawn in the
+  // step.
+  constexpr int G = group_steps(K);
   const uint32_t refill_mask = (uint32_t)j.clock_win - 1u;
-  for (uint32_t row0 = 0; __any(active); row0 += 4) {
-    if ((row0 & refill_mask) == 0u) src.refill(f, active, T32);
-    float zq[4 * K];
-    src.rows4(row0, active, zq);
+  for (uint32_t row0 = 0; __any(active); row0 += G) {
+    if ((row0 & refill_mask) < (uint32_t)G) src.refill(f, active, T32);
+    float zq[G * K];
+    src.rows(row0, active, zq);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < G; ++q) {
       if (active && steps >= max_steps) {
         overflow = 1;
         active = false;
