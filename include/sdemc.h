@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SDEMC_ABI_VERSION 4
+#define SDEMC_ABI_VERSION 5
 #define SDEMC_MAX_DIM 4
 #define SDEMC_CHUNK 64 /* paths per reduction chunk (one wavefront) */
 
@@ -66,7 +66,7 @@ enum sdemc_noise {
                                    (small-jump Gaussian, solvers.py:198-201)          */
   SDEMC_NOISE_INDEP_FULL = 2,   /* 'indep' in the diffusion solver: B is a second
                                    correlated BM independent of W (solvers.py:80-81)   */
-  SDEMC_NOISE_HESTON = 3        /* Heston (sde.py:258-279, schemes.py:16-21): dim 2,
+  SDEMC_NOISE_HESTON = 3,       /* Heston (sde.py:258-279, schemes.py:16-21): dim 2,
                                    x = (S, v), correlated 'diag' increments dW;
                                    S += a1[0] S dt + sqrt(v) S dW_0 and sqrt(v_new)
                                    is the positive root of the drift-implicit
@@ -74,6 +74,14 @@ enum sdemc_noise {
                                    - (v + (kappa theta - xi^2/2) dt) = 0
                                    (helpers.py:36-48); jp = {kappa, theta, xi};
                                    no jumps, uniform grid, 2 kappa theta > xi^2  */
+  SDEMC_NOISE_GENERAL = 4       /* 'general' (sde.py:31-32, schemes.py:12): a
+                                   (dim, brown_dim) diffusion matrix applied to
+                                   the correlated brown_dim-dimensional BM,
+                                   B(x)_ij = bm0[i*4+j] + bm1[i*4+j] * x_i:
+                                   x_i <- x_i + a_i(x) dt + sum_j B(x)_ij dW_j,
+                                   dW = chol * z * sqrt(dt) with chol
+                                   brown_dim x brown_dim; b0, b1, c0, c1
+                                   unused; uniform grid                      */
 };
 
 enum sdemc_jump_law {
@@ -99,6 +107,10 @@ typedef struct sdemc_model {
   double ax[SDEMC_MAX_DIM]; /* drift of component i gains ax[i] * x_0 ('diag' noise,
                                ax[0] = 0): AsianWrapper's running integral
                                dA = S dt (sde.py:378-406)                      */
+  int32_t brown_dim;        /* 'general' noise: Brownian dimension 1..4        */
+  int32_t reserved;
+  double bm0[SDEMC_MAX_DIM * SDEMC_MAX_DIM]; /* 'general' noise: B(x) = bm0 + diag(x) bm1, */
+  double bm1[SDEMC_MAX_DIM * SDEMC_MAX_DIM]; /* row-major (dim, brown_dim) in stride 4     */
 } sdemc_model;
 
 /* ---- payoff (options.py:156-321) ---------------------------------------- */
@@ -162,14 +174,17 @@ typedef struct sdemc_path_out {
   float* left_dev;      /* adapted only, [S+1][dim][P] pre-jump states              */
   float* jumps_dev;     /* adapted only, [S+1][dim][P] applied jump marks           */
   float* normals_dev;   /* [S][K][P] increments used (scaled and correlated);
-                           K = dim ('diag'), 2*dim ('indep': column-major pairs)    */
+                           K = dim ('diag'), 2*dim ('indep': column-major pairs),
+                           4 ('general': dW_0..3, zero beyond brown_dim)            */
 } sdemc_path_out;
 
 /* ---- host-supplied random inputs (fixed-increment mode) ------------------
  * Uniform grid:  dw     [N][K][P] increments, already scaled by sqrt(dt) and
  *                       correlated (the reference's `normals`, solvers.py:206);
  *                       K = dim ('diag'), dim + 1 ('indep' common: the last row
- *                       is the shared small-jump increment), 2*dim ('indep' full)
+ *                       is the shared small-jump increment), 2*dim ('indep' full),
+ *                       4 ('general': dW_0..3; rows beyond brown_dim meet zero
+ *                       matrix columns)
  *                counts [N][P] jumps arriving in step n
  *                jump_off [P+1] CSR offsets into jump_size (int64)
  *                jump_size marks J, path-major in arrival order

@@ -244,3 +244,45 @@ def multilevel_adapted(model, T, fine, coarse, bs, fixed, exact_jumps=False, pla
         x_coarse = x_coarse + (model.j0 + model.j1 * yc) * cur
         jump_idx = torch.where(hit, jump_idx + 1, jump_idx)
     return x_fine, x_coarse, iters
+
+
+def general_uniform(x0, a0, a1, B0, B1, chol, T, z, coarse_factor=0):
+    """DiffusionSolver.solve / multilevel_solve (solvers.py:68-119) for a
+    'general' model in fp64 torch, on fixed standard normals z [N][K][bs]
+    (K >= brown_dim; columns past brown_dim unused).
+
+    The increments are corr_normals = L (z sqrt(h)) (solvers.py:51-56) and the
+    step is the matrix branch of EulerScheme.step (schemes.py:12) as intended,
+    x + a(x) h + torch.matmul(B(x), dW), with a(x) = a0 + a1 x and
+    B(x) = B0 + x[..., None] B1 of shape (bs, dim, brown_dim).  With
+    coarse_factor M >= 2 the coarse path takes steps of M h with the sum of
+    its M fine increments (solvers.py:104-119).  Returns x_fine [bs, dim]
+    (and x_coarse)."""
+    f64 = torch.float64
+    L = torch.as_tensor(np.asarray(chol), dtype=f64)
+    nb = L.shape[0]
+    B0 = torch.as_tensor(np.asarray(B0), dtype=f64)
+    B1 = torch.as_tensor(np.asarray(B1), dtype=f64)
+    a0 = torch.as_tensor(np.asarray(a0), dtype=f64)
+    a1 = torch.as_tensor(np.asarray(a1), dtype=f64)
+    z = torch.as_tensor(np.asarray(z), dtype=f64)
+    N, bs = z.shape[0], z.shape[2]
+    h = T / N
+
+    def step(x, dt, dW):
+        B = B0 + x.unsqueeze(-1) * B1
+        return x + (a0 + a1 * x) * dt + torch.matmul(B, dW.unsqueeze(-1)).squeeze(-1)
+
+    x = torch.as_tensor(np.asarray(x0), dtype=f64).unsqueeze(0).repeat(bs, 1)
+    xc = x.clone()
+    M = int(coarse_factor)
+    run = torch.zeros((bs, nb), dtype=f64)
+    for n in range(N):
+        dW = torch.matmul(L, z[n, :nb] * math.sqrt(h)).T  # (bs, brown_dim)
+        x = step(x, h, dW)
+        if M:
+            run = run + dW
+            if (n + 1) % M == 0:
+                xc = step(xc, M * h, run)
+                run = torch.zeros_like(run)
+    return (x, xc) if M else x

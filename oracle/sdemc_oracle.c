@@ -138,6 +138,7 @@ ORACLE_API double oracle_normal(uint64_t seed, uint64_t path, uint32_t i) {
 typedef struct {
   const sdemc_job* job;
   int D, K, noise;
+  int R; /* entries of dW per step: D, or 4 for 'general' (the Brownian increments) */
   double L[D_MAX][D_MAX];
   float lv_y1, lv_y2, lv_y3;
   float inv_rate, n_over_T;
@@ -153,8 +154,16 @@ static void model_init(model_t* m, const sdemc_job* job) {
   m->K = (m->noise == SDEMC_NOISE_DIAG || m->noise == SDEMC_NOISE_HESTON)
              ? m->D
              : (m->noise == SDEMC_NOISE_INDEP_COMMON ? m->D + 1 : 2 * m->D);
+  m->R = m->D;
+  /* 'general': 4 normals per step whatever brown_dim, the factor zero-padded */
+  int nb = D_MAX;
+  if (m->noise == SDEMC_NOISE_GENERAL) {
+    m->K = m->R = D_MAX;
+    nb = job->model.brown_dim;
+  }
   for (int i = 0; i < D_MAX; ++i)
-    for (int j = 0; j < D_MAX; ++j) m->L[i][j] = job->model.chol[i * D_MAX + j];
+    for (int j = 0; j < D_MAX; ++j)
+      m->L[i][j] = (i < nb && j < nb) ? job->model.chol[i * D_MAX + j] : 0.0;
   m->lv_y1 = (float)job->model.jp[6];
   m->lv_y2 = (float)job->model.jp[7];
   m->lv_y3 = (float)job->model.jp[8];
@@ -219,6 +228,20 @@ static void euler(const model_t* m, double* x, double dt, const double* dW, cons
     x[1] = y * y;
     return;
   }
+  if (m->noise == SDEMC_NOISE_GENERAL) {
+    /* x + a(x) dt + B(x) dW with the (dim, brown_dim) matrix
+     * B(x)_ij = bm0_ij + bm1_ij x_i: the intended torch.matmul of
+     * schemes.py:12 on the correlated increments */
+    double xo[D_MAX];
+    memcpy(xo, x, sizeof(double) * m->D);
+    for (int i = 0; i < m->D; ++i) {
+      double diff = 0.0;
+      for (int j = 0; j < md->brown_dim; ++j)
+        diff += (md->bm0[i * D_MAX + j] + md->bm1[i * D_MAX + j] * xo[i]) * dW[j];
+      x[i] = xo[i] + (md->a0[i] + md->a1[i] * xo[i]) * dt + diff;
+    }
+    return;
+  }
   const double s0 = x[0]; /* pre-step x_0: AsianWrapper's integrand (sde.py:389-390) */
   for (int i = 0; i < m->D; ++i) {
     double a = md->a0[i] + md->a1[i] * x[i] + (i > 0 ? md->ax[i] * s0 : 0.0);
@@ -239,6 +262,15 @@ static void add_jump(const model_t* m, double* x, const double* y, double J) {
  * the diffusion solver, a second correlated vector (solvers.py:80-81) */
 static void correlate(const model_t* m, const double* z, double sq, double* dW, double* dB) {
   int D = m->D;
+  if (m->noise == SDEMC_NOISE_GENERAL) {
+    for (int i = 0; i < D_MAX; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j <= i; ++j) acc += m->L[i][j] * (z[j] * sq);
+      dW[i] = acc;
+      dB[i] = 0.0;
+    }
+    return;
+  }
   for (int i = 0; i < D; ++i) {
     double acc = 0.0;
     for (int j = 0; j <= i; ++j) acc += m->L[i][j] * (z[j] * sq);
@@ -373,6 +405,13 @@ static int poisson_marks(const model_t* m, src_t* s, int32_t n, double* marks, i
 static void fixed_row(const model_t* m, src_t* s, int32_t n, double* dW, double* dB) {
   int D = m->D, K = m->K;
   const float* row = s->dw + (int64_t)n * K * s->P + s->p;
+  if (m->noise == SDEMC_NOISE_GENERAL) {
+    for (int i = 0; i < D_MAX; ++i) {
+      dW[i] = row[(int64_t)i * s->P];
+      dB[i] = 0.0;
+    }
+    return;
+  }
   for (int i = 0; i < D; ++i) dW[i] = row[(int64_t)i * s->P];
   for (int i = 0; i < D; ++i) dB[i] = 0.0;
   if (m->noise == SDEMC_NOISE_INDEP_COMMON)
@@ -480,7 +519,7 @@ static void uniform_coupled(const model_t* m, src_t* s, double* xf, double* xc) 
         for (int k = 0; k < K; ++k) z[k] = oracle_normal(s->seed, s->path, (uint32_t)(n * K + k));
         correlate(m, z, sq, dW, dB);
       }
-      for (int i = 0; i < D; ++i) {
+      for (int i = 0; i < m->R; ++i) {
         sW[i] += dW[i];
         sB[i] += dB[i];
       }

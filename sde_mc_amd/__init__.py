@@ -9,8 +9,8 @@ reached through the C ABI of ``lib/libsdemc.so`` (include/sdemc.h).
 from .version import __version__
 from .helpers import (partition, solve_quadratic, mc_estimates, remove_steps, get_corr_matrix,
                       ceil_mult, get_jump_comp, sample_cov)
-from .sde import (Sde, DiffusionSde, Gbm, LogGbm, DoubleGbm, Heston, LogNormalJumpsSde, Merton,
-                  AsianWrapper, KernelModel)
+from .sde import (Sde, DiffusionSde, Gbm, LogGbm, DoubleGbm, GeneralSde, MultiFactorGbm, Heston,
+                  LogNormalJumpsSde, Merton, AsianWrapper, KernelModel)
 from .levy import UNIFORM_TOL, InverseCdf, Levy, LevySde, ExampleLevy, ExpExampleLevy, Levy2d
 from .schemes import EulerScheme, HestonScheme
 from .options import (bs_binary_aon, bs_call, merton_call, bs_digital_call, bs_asian_call, Option,

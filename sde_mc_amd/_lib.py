@@ -14,11 +14,11 @@ CHUNK = 64
 
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_WORKSPACE, ERR_OVERFLOW, ERR_DEVICE = range(7)
 
-ABI_VERSION = 4   # SDEMC_ABI_VERSION these structures follow
+ABI_VERSION = 5   # SDEMC_ABI_VERSION these structures follow
 # stats flag bits (enum sdemc_flag)
 FLAG_OVERFLOW, FLAG_JUMP_CSR, FLAG_CLASS, FLAG_TABLE = 1, 2, 4, 8
 
-NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON = 0, 1, 2, 3
+NOISE_DIAG, NOISE_INDEP_COMMON, NOISE_INDEP_FULL, NOISE_HESTON, NOISE_GENERAL = 0, 1, 2, 3, 4
 JUMP_LOGNORMAL, JUMP_LEVY = 0, 1
 GRID_UNIFORM, GRID_ADAPTED = 0, 1
 (PAYOFF_EURO_CALL, PAYOFF_EURO_PUT, PAYOFF_BINARY_AON, PAYOFF_DIGITAL, PAYOFF_BASKET_ARITH,
@@ -35,7 +35,9 @@ class Model(ctypes.Structure):
         ("x0", _D4), ("a0", _D4), ("a1", _D4), ("b0", _D4), ("b1", _D4),
         ("c0", _D4), ("c1", _D4), ("j0", _D4), ("j1", _D4),
         ("chol", ctypes.c_double * (MAX_DIM * MAX_DIM)), ("jp", ctypes.c_double * 12),
-        ("ax", _D4),
+        ("ax", _D4), ("brown_dim", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("bm0", ctypes.c_double * (MAX_DIM * MAX_DIM)),
+        ("bm1", ctypes.c_double * (MAX_DIM * MAX_DIM)),
     ]
 
 

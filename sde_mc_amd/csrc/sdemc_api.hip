@@ -54,7 +54,7 @@ int validate_job(const sdemc_job* job) {
   if (!job) return SDEMC_ERR_ARG;
   const sdemc_model& m = job->model;
   if (m.dim < 1 || m.dim > SDEMC_MAX_DIM) return SDEMC_ERR_UNSUPPORTED;
-  if (m.noise < SDEMC_NOISE_DIAG || m.noise > SDEMC_NOISE_HESTON) return SDEMC_ERR_UNSUPPORTED;
+  if (m.noise < SDEMC_NOISE_DIAG || m.noise > SDEMC_NOISE_GENERAL) return SDEMC_ERR_UNSUPPORTED;
   if (m.jump_law != SDEMC_JUMP_LOGNORMAL && m.jump_law != SDEMC_JUMP_LEVY)
     return SDEMC_ERR_UNSUPPORTED;
   if (m.exact_jumps != 0 && m.exact_jumps != 1) return SDEMC_ERR_ARG;
@@ -65,7 +65,8 @@ int validate_job(const sdemc_job* job) {
         !finite(m.j1[i]))
       return SDEMC_ERR_ARG;
     for (int j = 0; j < m.dim; ++j)
-      if (!finite(m.chol[i * SDEMC_MAX_DIM + j])) return SDEMC_ERR_ARG;
+      if (m.noise != SDEMC_NOISE_GENERAL && !finite(m.chol[i * SDEMC_MAX_DIM + j]))
+        return SDEMC_ERR_ARG;
     if (!finite(m.ax[i])) return SDEMC_ERR_ARG;
     if (m.ax[i] != 0.0 && (i == 0 || m.noise != SDEMC_NOISE_DIAG)) return SDEMC_ERR_UNSUPPORTED;
   }
@@ -102,8 +103,21 @@ int validate_job(const sdemc_job* job) {
   if (job->n_paths > (1ull << 37)) return SDEMC_ERR_ARG;
   // bounded expected jumps per path keeps every per-step jump loop short
   if (m.rate * job->T > 1.0e5) return SDEMC_ERR_ARG;
-  if (m.noise == SDEMC_NOISE_INDEP_FULL && job->grid == SDEMC_GRID_ADAPTED)
+  if ((m.noise == SDEMC_NOISE_INDEP_FULL || m.noise == SDEMC_NOISE_GENERAL) &&
+      job->grid == SDEMC_GRID_ADAPTED)
     return SDEMC_ERR_UNSUPPORTED;
+  if (m.noise == SDEMC_NOISE_GENERAL) {
+    // the (dim, brown_dim) matrices and the brown_dim x brown_dim Cholesky
+    // factor; entries outside them are ignored
+    if (m.brown_dim < 1 || m.brown_dim > SDEMC_MAX_DIM) return SDEMC_ERR_UNSUPPORTED;
+    for (int i = 0; i < m.brown_dim; ++i)
+      for (int j = 0; j < m.brown_dim; ++j)
+        if (!finite(m.chol[i * SDEMC_MAX_DIM + j])) return SDEMC_ERR_ARG;
+    for (int i = 0; i < m.dim; ++i)
+      for (int j = 0; j < m.brown_dim; ++j)
+        if (!finite(m.bm0[i * SDEMC_MAX_DIM + j]) || !finite(m.bm1[i * SDEMC_MAX_DIM + j]))
+          return SDEMC_ERR_ARG;
+  }
   if (m.noise == SDEMC_NOISE_HESTON) {
     // (S, v) with no jumps on the uniform grid; the Feller condition keeps the
     // implicit variance equation's constant term positive (sde.py:260)
@@ -293,6 +307,7 @@ static int dispatch_rng(const sdemc_job& job, const LaunchArgs& a) {
     case SDEMC_NOISE_INDEP_COMMON: return launch_rng_levy(job, a);
     case SDEMC_NOISE_INDEP_FULL: return launch_rng_double(job, a);
     case SDEMC_NOISE_HESTON: return launch_rng_heston(job, a);
+    case SDEMC_NOISE_GENERAL: return launch_rng_general(job, a);
     default: return SDEMC_ERR_UNSUPPORTED;
   }
 }
@@ -303,6 +318,7 @@ static int dispatch_fixed(const sdemc_job& job, const sdemc_fixed_in& in, const 
     case SDEMC_NOISE_INDEP_COMMON: return launch_fixed_levy(job, in, a);
     case SDEMC_NOISE_INDEP_FULL: return launch_fixed_double(job, in, a);
     case SDEMC_NOISE_HESTON: return launch_fixed_heston(job, in, a);
+    case SDEMC_NOISE_GENERAL: return launch_fixed_general(job, in, a);
     default: return SDEMC_ERR_UNSUPPORTED;
   }
 }
@@ -375,7 +391,7 @@ int sdemc_cv_simulate(const sdemc_job* job, const sdemc_cv_spec* cv, double* sta
   if (st != SDEMC_OK) return st;
   if (!cv || !stats_dev || !workspace_dev) return SDEMC_ERR_ARG;
   const sdemc_model& m = job->model;
-  if (job->coarse_factor != 0) return SDEMC_ERR_UNSUPPORTED;
+  if (job->coarse_factor != 0 || m.noise == SDEMC_NOISE_GENERAL) return SDEMC_ERR_UNSUPPORTED;
   const bool adapted = job->grid == SDEMC_GRID_ADAPTED;
   if (adapted) {
     if (m.noise != SDEMC_NOISE_DIAG && m.noise != SDEMC_NOISE_INDEP_COMMON)

@@ -32,7 +32,12 @@
 
 namespace sdemc {
 
-enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2, FAM_HESTON = 3 };
+enum { FAM_MERTON = 0, FAM_LEVY = 1, FAM_DOUBLE = 2, FAM_HESTON = 3, FAM_GENERAL = 4 };
+
+// 'general' noise draws SDEMC_MAX_DIM normals per step whatever its Brownian
+// dimension: the Cholesky factor and the diffusion matrices are zero-padded,
+// so the unused normals contribute exact zeros.
+constexpr int GENERAL_K = SDEMC_MAX_DIM;
 
 // Families whose increments are D correlated normals and no second driver.
 template <int FAM>
@@ -224,6 +229,9 @@ struct KModel {
   float inv_rate;  // 1 / lambda (fp32 jump clock)
   float hv_kappa, hv_drift, hv_xi;  // Heston variance: kappa, kappa theta - xi^2/2, xi
   float ax[SDEMC_MAX_DIM];          // drift of component i gains ax[i] x_0 ('diag')
+  // 'general' noise: B(x)_ij = bm0[i][j] + bm1[i][j] x_i, zero outside
+  // (dim, brown_dim); L is then the brown_dim x brown_dim factor, zero-padded
+  float bm0[SDEMC_MAX_DIM][SDEMC_MAX_DIM], bm1[SDEMC_MAX_DIM][SDEMC_MAX_DIM];
 };
 
 struct KPayoff {
@@ -319,6 +327,15 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
     f.j1[i] = (float)m.j1[i];
     for (int q = 0; q < SDEMC_MAX_DIM; ++q) f.L[i][q] = (float)m.chol[i * SDEMC_MAX_DIM + q];
     f.ax[i] = (float)m.ax[i];
+  }
+  const bool general = m.noise == SDEMC_NOISE_GENERAL;
+  for (int i = 0; i < SDEMC_MAX_DIM; ++i) {
+    for (int q = 0; q < SDEMC_MAX_DIM; ++q) {
+      const bool in_b = general && i < m.dim && q < m.brown_dim;
+      f.bm0[i][q] = in_b ? (float)m.bm0[i * SDEMC_MAX_DIM + q] : 0.0f;
+      f.bm1[i][q] = in_b ? (float)m.bm1[i * SDEMC_MAX_DIM + q] : 0.0f;
+      if (general && (i >= m.brown_dim || q >= m.brown_dim)) f.L[i][q] = 0.0f;
+    }
   }
   f.exact = m.exact_jumps;
   f.law = m.jump_law;
@@ -573,6 +590,18 @@ __device__ __forceinline__ void euler(const KModel& f, float (&x)[D], float dt,
     x[1] = y * y;
     return;
   }
+  if constexpr (FAM == FAM_GENERAL) {
+    // dW = bm0 w, dB = bm1 w (general_apply), so row i of B(x) w is
+    // dW_i + x_i dB_i with x_i the pre-step state
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const float xi = x[i];
+      const float a = __fmaf_rn(f.a1[i], xi, f.a0[i]);
+      const float t = __fmaf_rn(a, dt, xi);
+      x[i] = t + __fmaf_rn(xi, dB[i], dW[i]);
+    }
+    return;
+  }
   const float s0 = x[0];  // pre-step x_0 for the cross drift ax[i] x_0
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -600,14 +629,57 @@ __device__ __forceinline__ void add_jump(const KModel& f, float (&x)[D], const f
 // Number of normals drawn per step.
 template <int D, int FAM>
 struct NK {
-  static constexpr int K = DiagNoise<FAM>::value ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+  static constexpr int K = FAM == FAM_GENERAL
+                               ? GENERAL_K
+                               : (DiagNoise<FAM>::value ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D));
 };
+
+// 'general' noise: the correlated increments w = L (z sqrt(dt)) of the
+// Brownian motion (solvers.py:51-56), all GENERAL_K of them (zero beyond
+// brown_dim), ...
+__device__ __forceinline__ void general_w(const KModel& f, const float* z, float sq,
+                                          float (&w)[GENERAL_K]) {
+  float zs[GENERAL_K];
+#pragma unroll
+  for (int q = 0; q < GENERAL_K; ++q) zs[q] = z[q] * sq;
+#pragma unroll
+  for (int i = 0; i < GENERAL_K; ++i) {
+    float acc = f.L[i][0] * zs[0];
+#pragma unroll
+    for (int q = 1; q <= i; ++q) acc = __fmaf_rn(f.L[i][q], zs[q], acc);
+    w[i] = acc;
+  }
+}
+
+// ... and the two products the Euler step needs: e = bm0 w and g = bm1 w,
+// with which B(x) w = e + diag(x) g.
+template <int D>
+__device__ __forceinline__ void general_apply(const KModel& f, const float (&w)[GENERAL_K],
+                                              float (&e)[D], float (&g)[D]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    float a = f.bm0[i][0] * w[0], b = f.bm1[i][0] * w[0];
+#pragma unroll
+    for (int q = 1; q < GENERAL_K; ++q) {
+      a = __fmaf_rn(f.bm0[i][q], w[q], a);
+      b = __fmaf_rn(f.bm1[i][q], w[q], b);
+    }
+    e[i] = a;
+    g[i] = b;
+  }
+}
 
 // Correlated, scaled increments from K standard normals:
 //   dW = L (z[0:D] sqrt(dt)),  dB = z[D] sqrt(dt) (common) or L (z[D:2D] sqrt(dt)).
 template <int D, int FAM>
 __device__ __forceinline__ void correlate(const KModel& f, const float* z, float sq,
                                           float (&dW)[D], float (&dB)[D]) {
+  if constexpr (FAM == FAM_GENERAL) {
+    float w[GENERAL_K];
+    general_w(f, z, sq, w);
+    general_apply<D>(f, w, dW, dB);
+    return;
+  }
   float zs[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) zs[j] = z[j] * sq;
@@ -1298,7 +1370,14 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
           }
           if (st.normals) {
             const int64_t rn = (int64_t)n * D * (DiagNoise<FAM>::value ? 1 : 2);
-            if (DiagNoise<FAM>::value) {
+            if constexpr (FAM == FAM_GENERAL) {
+              // the correlated increments w, as fixed-increment rows take them
+              float w[GENERAL_K];
+              general_w(f, &z[s * K], sq, w);
+              const int64_t rg = (int64_t)n * GENERAL_K;
+#pragma unroll
+              for (int q = 0; q < GENERAL_K; ++q) st.normals[(rg + q) * st.P + c.p] = w[q];
+            } else if (DiagNoise<FAM>::value) {
 #pragma unroll
               for (int i = 0; i < D; ++i) st.normals[(rn + i) * st.P + c.p] = dW[i];
             } else {
@@ -1376,12 +1455,21 @@ __device__ __forceinline__ void path_uniform_rng_coupled(const KJob& j, const Pa
 // common: the last row is the shared increment), 2D ('indep' full).
 template <int D, int FAM>
 struct NKF {
-  static constexpr int K = DiagNoise<FAM>::value ? D : (FAM == FAM_LEVY ? D + 1 : 2 * D);
+  static constexpr int K = NK<D, FAM>::K;
 };
 
+// One step's row of fixed increments: dW (and dB) as the kernels' Euler step
+// takes them; 'general' rows hold the GENERAL_K correlated increments w.
 template <int D, int FAM>
-__device__ __forceinline__ void fixed_row(const float* row, int64_t P, float (&dW)[D],
-                                          float (&dB)[D]) {
+__device__ __forceinline__ void fixed_row(const KModel& f, const float* row, int64_t P,
+                                          float (&dW)[D], float (&dB)[D]) {
+  if constexpr (FAM == FAM_GENERAL) {
+    float w[GENERAL_K];
+#pragma unroll
+    for (int q = 0; q < GENERAL_K; ++q) w[q] = row[(int64_t)q * P];
+    general_apply<D>(f, w, dW, dB);
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < D; ++i) dW[i] = row[(int64_t)i * P];
   if (FAM == FAM_LEVY) {
@@ -1451,7 +1539,7 @@ __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int6
     int cnt[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      fixed_row<D, FAM>(dwp + (int64_t)(n + g) * KF * P, P, dW[g], dB[g]);
+      fixed_row<D, FAM>(f, dwp + (int64_t)(n + g) * KF * P, P, dW[g], dB[g]);
       cnt[g] = cntp ? (int)cntp[(int64_t)(n + g) * P] : 0;
     }
 #pragma unroll
@@ -1460,7 +1548,7 @@ __device__ void path_uniform_fixed(const KJob& j, const sdemc_fixed_in& in, int6
   }
   for (; n < N; ++n) {
     float dW[D], dB[D];
-    fixed_row<D, FAM>(dwp + (int64_t)n * KF * P, P, dW, dB);
+    fixed_row<D, FAM>(f, dwp + (int64_t)n * KF * P, P, dW, dB);
     const int cnt = cntp ? (int)cntp[(int64_t)n * P] : 0;
     fixed_step<D, FAM>(j, in, x, dW, dB, cnt, off, bad);
   }
@@ -1554,9 +1642,9 @@ struct UniFixedSrc {
     left = 0;
     bad = 0;
   }
-  __device__ __forceinline__ void incr(const KJob&, int32_t n, float (&dW)[D], float (&dB)[D]) {
+  __device__ __forceinline__ void incr(const KJob& j, int32_t n, float (&dW)[D], float (&dB)[D]) {
     constexpr int KF = NKF<D, FAM>::K;
-    fixed_row<D, FAM>(in->dw_dev + (int64_t)n * KF * P + p, P, dW, dB);
+    fixed_row<D, FAM>(j.m, in->dw_dev + (int64_t)n * KF * P + p, P, dW, dB);
     cur_n = n;
     left = in->counts_dev ? (int)in->counts_dev[(int64_t)n * P + p] : 0;
   }

@@ -219,7 +219,7 @@ int launch_rng_d(const sdemc_job& sj, const LaunchArgs& a) {
                        job, a.partials, a.cflags, a.out, nchunks);                            \
   else                                                                                        \
     SDEMC_L(SDEMC_GRID_UNIFORM, false, S)
-  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON) {
+  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON || FAM == FAM_GENERAL) {
     if (adapted) return SDEMC_ERR_UNSUPPORTED;
     if (coupled) SDEMC_L(SDEMC_GRID_UNIFORM, true, false);
     else if (a.store) { SDEMC_LU(true); }
@@ -258,7 +258,7 @@ int launch_fixed_d(const sdemc_job& sj, const sdemc_fixed_in& in, const LaunchAr
 #define SDEMC_L(G, C)                                                                        \
   hipLaunchKernelGGL((k_fixed<D, FAM, G, C>), g, b, 0, a.stream, job, in, a.partials, a.cflags, \
                      a.out, nchunks)
-  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON) {
+  if constexpr (FAM == FAM_DOUBLE || FAM == FAM_HESTON || FAM == FAM_GENERAL) {
     if (adapted) return SDEMC_ERR_UNSUPPORTED;
     if (coupled) SDEMC_L(SDEMC_GRID_UNIFORM, true);
     else SDEMC_L(SDEMC_GRID_UNIFORM, false);
@@ -308,5 +308,7 @@ int launch_fixed_levy(const sdemc_job& job, const sdemc_fixed_in& in, const Laun
 int launch_fixed_double(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
 int launch_rng_heston(const sdemc_job& job, const LaunchArgs& a);
 int launch_fixed_heston(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
+int launch_rng_general(const sdemc_job& job, const LaunchArgs& a);
+int launch_fixed_general(const sdemc_job& job, const sdemc_fixed_in& in, const LaunchArgs& a);
 
 }  // namespace sdemc

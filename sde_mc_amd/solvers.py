@@ -134,6 +134,22 @@ class SdeSolver(ABC):
             pay = option(x) * discount
         return pay, x
 
+    def _normal_rows(self):
+        """Rows per step of the stored increments: dim ('diag'), the padded
+        Brownian dimension ('general'), else (dW, dB) pairs."""
+        kind, d = self.sde.diffusion_struct, self.sde.dim
+        return d if kind == 'diag' else (L.MAX_DIM if kind == 'general' else 2 * d)
+
+    def _normals_out(self, nrm):
+        """Stored increments [S][K][bs] in the reference's layout: (bs, S, dim)
+        for 'diag', (bs, S, brown_dim) for 'general', (bs, S, dim, 2) else."""
+        kind, d = self.sde.diffusion_struct, self.sde.dim
+        if kind == 'diag':
+            return nrm.permute(2, 0, 1)
+        if kind == 'general':
+            return nrm[:, :self.sde.brown_dim].permute(2, 0, 1)
+        return nrm.reshape(nrm.shape[0], d, 2, nrm.shape[-1]).permute(3, 0, 1, 2)
+
     def _materialise(self, bs, low_storage):
         """One stored solve: paths [S+1][d][P] and the aux arrays, on device."""
         dev = E.require_device()
@@ -141,7 +157,7 @@ class SdeSolver(ABC):
         job, _ = self.make_job(bs)
         adapted = job.grid == L.GRID_ADAPTED
         rows = job.max_steps if adapted else job.steps
-        kn = d if self.sde.diffusion_struct == 'diag' else 2 * d
+        kn = self._normal_rows()
         paths = torch.empty((rows + 1, d, bs), dtype=torch.float32, device=dev)
         nsteps = torch.empty(bs, dtype=torch.int32, device=dev)
         out = L.PathOut()
@@ -172,13 +188,7 @@ class DiffusionSolver(SdeSolver):
         job, rows, paths, nsteps, aux = self._materialise(bs, low_storage=False)
         od = self._out_device()
         p = paths.permute(2, 0, 1).to(od)
-        nrm = aux['normals']
-        d = self.sde.dim
-        if self.sde.diffusion_struct == 'diag':
-            normals = nrm.permute(2, 0, 1)
-        else:
-            normals = nrm.reshape(rows, d, 2, bs).permute(3, 0, 1, 2)
-        return p, normals.to(od)
+        return p, self._normals_out(aux['normals']).to(od)
 
     def multilevel_solve(self, bs, levels, return_normals=False):
         """Coupled fine/coarse pair (solvers.py:90-119).  Returns (bs, 2, dim) tensors
@@ -248,11 +258,7 @@ class JumpDiffusionSolver(SdeSolver):
             left_paths = None
         jumps = aux['jumps']
         jumps[total + 1:] = 0.0
-        nrm = aux['normals']
-        if self.sde.diffusion_struct == 'diag':
-            normals = nrm.permute(2, 0, 1)
-        else:
-            normals = nrm.reshape(nrm.shape[0], d, 2, bs).permute(3, 0, 1, 2)
+        normals = self._normals_out(aux['normals'])
         left_out = left_paths.to(od) if left_paths is not None else None
         return out_paths, (normals.to(od), time_paths.to(od), left_out, total,
                            jumps.permute(2, 0, 1).to(od))

@@ -47,6 +47,8 @@ def job_for(sde, T, steps, n_paths, grid='uniform', seed=1234, path_offset=0, co
 
 def n_normals(job):
     d, nz = job.model.dim, job.model.noise
+    if nz == L.NOISE_GENERAL:
+        return L.MAX_DIM
     if nz in (L.NOISE_DIAG, L.NOISE_HESTON):
         return d
     return d + 1 if nz == L.NOISE_INDEP_COMMON else 2 * d
@@ -112,10 +114,11 @@ def fixed_uniform(job, seed=0):
     d = job.model.dim
     K = n_normals(job)
     h = T / N
-    L_ = np.array([[job.model.chol[i * 4 + j] for j in range(d)] for i in range(d)])
+    nb = job.model.brown_dim if job.model.noise == L.NOISE_GENERAL else d
+    L_ = np.array([[job.model.chol[i * 4 + j] for j in range(nb)] for i in range(nb)])
     z = rng.standard_normal((N, K, P)) * math.sqrt(h)
-    dw = np.empty((N, K, P), dtype=np.float32)
-    dw[:, :d, :] = np.einsum('ij,njp->nip', L_, z[:, :d, :])
+    dw = np.zeros((N, K, P), dtype=np.float32)
+    dw[:, :nb, :] = np.einsum('ij,njp->nip', L_, z[:, :nb, :])
     if job.model.noise == L.NOISE_INDEP_COMMON:
         dw[:, d, :] = z[:, d, :]
     elif job.model.noise == L.NOISE_INDEP_FULL:

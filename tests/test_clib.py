@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_layout():
     lib = L.load()
-    assert lib.sdemc_abi_version() == L.ABI_VERSION == 4
+    assert lib.sdemc_abi_version() == L.ABI_VERSION == 5
     L.check_layout()
     assert lib.sdemc_sizeof_prepared_job() % 8 == 0
 
@@ -86,6 +86,38 @@ def test_prepare_rejects_bad_jobs(mutate, status):
     job = J.job_for(J.merton_sde(), 1.0, 500, 1000)
     mutate(job)
     assert _prepare(job)[0] == status
+
+
+def _general_job(**kw):
+    import torch
+    import sde_mc_amd as sm
+    sde = sm.GeneralSde(0.0, 0.02, torch.tensor([[0.1, 0.2, 0.0], [0.0, 0.1, -0.1]]),
+                        0.0, torch.ones(2), 2, 3)
+    return J.job_for(sde, 1.0, 50, 1000, **kw)
+
+
+def test_general_noise_validation():
+    # 'general' runs on the uniform grid in the generic batched class; the
+    # brown_dim, the adapted grid and non-finite matrix entries are refused
+    st, cs = _prepare(_general_job())
+    assert st == L.OK and cs == L.CLASS_GENERIC
+    assert _prepare(_general_job(coarse_factor=2))[0] == L.OK
+    job = _general_job()
+    job.grid = L.GRID_ADAPTED
+    assert _prepare(job)[0] == L.ERR_UNSUPPORTED
+    for bd in (0, 5):
+        job = _general_job()
+        job.model.brown_dim = bd
+        assert _prepare(job)[0] == L.ERR_UNSUPPORTED
+    job = _general_job()
+    job.model.bm1[1 * 4 + 2] = float('nan')
+    assert _prepare(job)[0] == L.ERR_ARG
+    job = _general_job()
+    job.model.bm1[1 * 4 + 3] = float('nan')  # outside (dim, brown_dim): ignored
+    assert _prepare(job)[0] == L.OK
+    job = _general_job()
+    job.model.chol[2 * 4 + 1] = float('inf')
+    assert _prepare(job)[0] == L.ERR_ARG
 
 
 def test_simulate_rejects_before_touching_device():
