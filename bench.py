@@ -244,10 +244,11 @@ def _fixed_fp64_error(dev, P=4096, N=500, seed=1):
 def valu_issue_ceiling(dev, iters=8192, reps=3):
     """Measured VALU issue rates: back-to-back instructions of one kind from 8
     resident wavefronts per SIMD on every CU (sdemc_valu_probe_dev), wavefront
-    instructions per second, best of `reps` launches each.  Kind 0 (v_fma_f32)
-    is the ceiling the path kernels' VALU instruction counts are priced at;
-    1 (v_xor_b32 + v_add_u32), 2 (v_mad_u64_u32 + v_xor_b32) and 3 (v_exp_f32
-    + 2 v_mul_f32) price Philox and Box-Muller."""
+    instructions per second, best of `reps` launches each: 0 (v_fma_f32),
+    1 (v_xor_b32 + v_add_u32) and 2 (v_mad_u64_u32 + v_xor_b32) check the
+    issue ceiling of one wavefront instruction per CU-cycle that the path
+    kernels' VALU counts are priced at; 3 (v_exp_f32 + 2 v_mul_f32) prices
+    the transcendentals; 4-6 time a Philox block and the normals of one."""
     import ctypes
     import torch
     from sde_mc_amd import _lib as L
@@ -296,12 +297,17 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
 # keyed by the launch shape's chunks per tile: nearly all of it is the chunk
 # partials' writes, which scatter more with one chunk per tile; and the
 # fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = {1: 1.134e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
+TRAFFIC_MIX_ROUND = {1: 1.142e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
 TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 # VALU instructions (wavefront-level, SQ_INSTS_VALU summed over the group
-# kernels) of one round of the served mix (128 requests, seed 0), from
-# rocprofv3 --pmc; profiles/k_batch_served_mix_128req_sq_counters.md.
-VALU_INSTR_MIX_ROUND = 3.95e10
+# kernels) of one round of the served mix (128 requests, seed 0) and the fp32
+# transcendentals among them (SQ_INSTS_VALU_TRANS_F32), from rocprofv3 --pmc;
+# profiles/k_batch_served_mix_128req_sq_counters.md.  A wave64 VALU
+# instruction holds its SIMD for 4 cycles, one per CU-cycle over the 4 SIMDs;
+# the transcendentals overlap the instructions after them, so the issue
+# figure counts the others.
+VALU_INSTR_MIX_ROUND = 2.783e10
+VALU_TRANS_MIX_ROUND = 2.066e9
 
 
 def _time_events(fn, reps):
@@ -725,7 +731,8 @@ def main():
     kern_s = kern_ms * 1e-3
     achieved = flops / kern_s
     mix_default = (args.requests, args.seed) == (128, 0)
-    valu_instr = VALU_INSTR_MIX_ROUND if mix_default else None
+    valu_instr = (VALU_INSTR_MIX_ROUND - VALU_TRANS_MIX_ROUND) if mix_default else None
+    issue_peak = ceiling["cus"] * ceiling["clock_mhz"] * 1e6  # wave instr / s
     out = {
         "metric": METRIC,
         "value": batched,
@@ -760,11 +767,13 @@ def main():
                      "valu_issue": {
                          "instr_per_round": valu_instr,
                          "achieved_wave_instr_per_s": valu_instr / kern_s if valu_instr else None,
-                         "ceiling_wave_instr_per_s": ceiling["wave_instr_per_s"],
-                         "frac": (valu_instr / kern_s / ceiling["wave_instr_per_s"]
-                                  if valu_instr else None),
-                         "source": "SQ_INSTS_VALU of a round (rocprofv3 --pmc) over the live "
-                                   "round time, against the live v_fma_f32 issue probe"},
+                         "ceiling_wave_instr_per_s": issue_peak,
+                         "frac": valu_instr / kern_s / issue_peak if valu_instr else None,
+                         "source": "non-transcendental VALU instructions of a round "
+                                   "(SQ_INSTS_VALU - SQ_INSTS_VALU_TRANS_F32, rocprofv3 "
+                                   "--pmc) over the live round time, against one wavefront "
+                                   "instruction per CU-cycle at the device clock (the "
+                                   "integer probes of valu_issue_ceiling reach 0.99-1.01)"},
                      "lane_ops": {"achieved": ops / kern_s / 1e12, "peak": PEAK_VALU_OPS / 1e12,
                                   "unit": "Top/s (VALU lane-ops counted from the algorithm, "
                                           "Philox 20 per word)",

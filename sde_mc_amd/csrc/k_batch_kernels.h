@@ -87,9 +87,19 @@ inline int32_t kjob_class_of(const KJob& k) {
                   k.pois_mode != 0);
 }
 
+// The generic class calls each case's chunk body as a function of its own,
+// once per 64-path chunk: the cases are compiled one by one instead of all
+// inlined into one kernel body (minutes of compile time less; the class runs
+// at 2 wavefronts per SIMD either way).
+template <int D, int FAM, int GRID, bool COUPLED>
+__device__ __attribute__((noinline)) void chunk_call(const KJob& job, int64_t chunk, double* part,
+                                                     int32_t* cf) {
+  chunk_body<D, FAM, GRID, COUPLED>(job, chunk, part, cf);
+}
+
 #define SDEMC_CASE(NOISE, D, FAM, GRID, C)                              \
   case ((NOISE * 4 + (D - 1)) * 2 + GRID) * 2 + C:                      \
-    chunk_body<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);        \
+    chunk_call<D, FAM, GRID, (C != 0)>(job, chunk, part, cf);        \
     return true;
 
 #define SDEMC_CASES_DG(NOISE, D, FAM, GRID) \
@@ -165,8 +175,12 @@ __device__ __forceinline__ void raise_fault(int32_t* fault, int lane) {
 // Persistent tile kernel of one class group.  hdr = {first segment,
 // segments, tiles}; tile indices come from this group's counter.  U is the
 // step unroll of the uniform single-level path (classes 0 and 4).
+// At least 4 wavefronts per SIMD: the class kernels of one family need at
+// most 64 VGPRs (8 wavefronts); the generic class, whose cases' register
+// needs go up to ~250 VGPRs for the 4-D two-driver family, is held to 128
+// (a few of its cases spill a little) instead of running at 2 per SIMD.
 template <int CLS, int U>
-__global__ __launch_bounds__(512) void k_batch(const KJob* __restrict__ jobs,
+__global__ __launch_bounds__(512, 4) void k_batch(const KJob* __restrict__ jobs,
                                                const sdemc_segment* __restrict__ segs,
                                                const int32_t* __restrict__ hdr,
                                                double* __restrict__ partials,

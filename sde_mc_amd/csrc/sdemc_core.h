@@ -397,15 +397,21 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   k.class_key = class_key_of(m.noise, m.dim, j.grid, j.coarse_factor > 0);
   k.pois_mode = 0;
   // A window's fill passes run when any lane of the wavefront jumps in it, so
-  // longer windows mean fewer passes, until a third jump in a window (beyond
-  // the two-slot queue, drawn in the step) becomes common: about half a jump
-  // per path per window measured best (against a quarter or one: 3-9%
-  // faster on 1-D Merton at lambda h = 0.004-0.016 and the 2-D Levy rainbow
-  // at 0.008-0.06).  The step loop rounds it up to its own stride.
+  // longer windows mean fewer passes per jump (the wavefront's passes follow
+  // the lane with the most jumps), until a jump beyond the queue (drawn in
+  // the step) becomes common.  Uniform grid, four-slot queue: about one jump
+  // per path per window measured best (against two slots at half a jump:
+  // 2.6-7% faster on the 2-D Levy rainbow at lambda h = 1/128 .. 1/32, 4-11%
+  // on 1-D Merton at 1/512 .. 1/32; six slots would cost occupancy).
+  // Adapted grid: its two jump slots are refilled every clock_win
+  // iterations, about half a jump per path apart.  The step loop rounds the
+  // window up to its own stride.
   {
     const double per_step = m.rate * j.T / (double)j.steps;
-    int32_t w = 128;
-    while (w > 4 && per_step * (double)w > 0.5) w >>= 1;
+    const bool uniform = j.grid == SDEMC_GRID_UNIFORM;
+    const double per_window = uniform ? 1.0 : 0.5;
+    int32_t w = uniform ? 512 : 128;
+    while (w > 4 && per_step * (double)w > per_window) w >>= 1;
     k.clock_win = w;
   }
   for (int q = 0; q < 16; ++q) {
@@ -1195,7 +1201,7 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   // wavefront from a list in LDS (pois_draw_window), and each step reads its
   // marks back.  Marks, order and arithmetic are those of the per-step form,
   // so results are unchanged.
-  constexpr int JQ = 2;
+  constexpr int JQ = 4;
   const bool clock = !pois && f.rate > 0.0;  // job-uniform
   PoisLds* const pl = pois ? &pois_lds() : nullptr;
   int32_t qk[JQ];
