@@ -399,17 +399,16 @@ inline void make_kjob(const sdemc_job& j, KJob& k) {
   // A window's fill passes run when any lane of the wavefront jumps in it, so
   // longer windows mean fewer passes per jump (the wavefront's passes follow
   // the lane with the most jumps), until a jump beyond the queue (drawn in
-  // the step) becomes common.  Uniform grid, four-slot queue: about one jump
-  // per path per window measured best (against two slots at half a jump:
-  // 2.6-7% faster on the 2-D Levy rainbow at lambda h = 1/128 .. 1/32, 4-11%
-  // on 1-D Merton at 1/512 .. 1/32; six slots would cost occupancy).
-  // Adapted grid: its two jump slots are refilled every clock_win
-  // iterations, about half a jump per path apart.  The step loop rounds the
-  // window up to its own stride.
+  // the step) becomes common.  Uniform grid, eight-slot queue in LDS: about
+  // four jumps per path per window measured best on the served mix (round
+  // 42.7 / 41.9 / 41.6 / 41.5 ms at one / two / three / four).  Adapted grid:
+  // its two jump slots are refilled every clock_win iterations, about half a
+  // jump per path apart.  The step loop rounds the window up to its own
+  // stride.
   {
     const double per_step = m.rate * j.T / (double)j.steps;
     const bool uniform = j.grid == SDEMC_GRID_UNIFORM;
-    const double per_window = uniform ? 1.0 : 0.5;
+    const double per_window = uniform ? 4.0 : 0.5;
     int32_t w = uniform ? 512 : 128;
     while (w > 4 && per_step * (double)w > per_window) w >>= 1;
     k.clock_win = w;
@@ -896,6 +895,18 @@ struct PoisLds {
   float x[POIS_XCAP];
 };
 
+// Arrival-clock jump queue of a wavefront (uniform grid): slot q of lane l
+// holds the step and the mark of the lane's q-th queued jump of the current
+// fill window, k = -1 past the last one.  It takes the space of PoisLds (a
+// kernel runs one jump mode); lane-contiguous, so every access is
+// conflict-free and each lane reads back only what it wrote itself.
+constexpr int CLOCK_JQ = 8;
+struct ClockLds {
+  int32_t k[CLOCK_JQ][WAVE];
+  float J[CLOCK_JQ][WAVE];
+};
+static_assert(sizeof(ClockLds) <= sizeof(PoisLds), "the clock queue shares the jump-list space");
+
 // One PoisLds per wavefront of the workgroup, in the launch's dynamic LDS
 // (the launchers size it as waves per block * sizeof(PoisLds)).
 extern __shared__ PoisLds pois_lds_dyn[];
@@ -1189,28 +1200,26 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
     kj = step_of(jc.tau, j.n_over_T, N);
   }
   // Jump draws off the step loop.  Arrival-clock mode: the marks of the
-  // coming steps are drawn ahead into a queue of JQ (step, mark) slots, in
-  // the order the per-step form applies them, by passes in which every lane
-  // of the wavefront takes its next jump at once, so the draw code runs
-  // about max-over-lanes times per window of clock_win steps (sized by the
-  // job's jump rate, rounded up to whole strides of G U) instead of once per
-  // step in which any lane jumps; a step checks only the queue's head, and
-  // jumps beyond the queue are taken inside the step loop by code compiled
-  // into the windows where some lane needs it.  Per-step Poisson mode: the
+  // coming steps are drawn ahead into a queue of CLOCK_JQ (step, mark) slots
+  // per lane in LDS, in the order the per-step form applies them, by passes
+  // in which every lane of the wavefront takes its next jump at once, so the
+  // draw code runs about max-over-lanes times per window of clock_win steps
+  // (sized by the job's jump rate, rounded up to whole strides of G U)
+  // instead of once per step in which any lane jumps; a step compares its
+  // index with the head's step, kept in a register, and a jump reads its
+  // mark and the next head from LDS (no register queue to shift); jumps
+  // beyond the queue are taken inside the step loop by code compiled into
+  // the windows where some lane needs it.  Per-step Poisson mode: the
   // jumping steps of each window of WIN steps are drawn by the whole
   // wavefront from a list in LDS (pois_draw_window), and each step reads its
   // marks back.  Marks, order and arithmetic are those of the per-step form,
   // so results are unchanged.
-  constexpr int JQ = 4;
   const bool clock = !pois && f.rate > 0.0;  // job-uniform
   PoisLds* const pl = pois ? &pois_lds() : nullptr;
-  int32_t qk[JQ];
-  float qJ[JQ];
-#pragma unroll
-  for (int q = 0; q < JQ; ++q) {
-    qk[q] = -1;
-    qJ[q] = 0.0f;
-  }
+  ClockLds* const cq = clock ? reinterpret_cast<ClockLds*>(&pois_lds()) : nullptr;
+  const uint32_t lane = pois ? 0u : __lane_id();
+  int32_t qh = 0;   // the lane's queue head
+  int32_t hk = -1;  // step of the head's jump, -1 when the queue is empty
 #pragma unroll
   for (int i = 0; i < D; ++i) x[i] = f.x0[i];
   if (STORE && lane_valid) {
@@ -1287,28 +1296,25 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
         for (int i = 0; i < D; ++i) xo[i] = x[i];
         uniform_step<D, FAM>(j, x, &z[s * K]);
         float jsum = 0.0f;
-        if (n == qk[0]) {
+        if (!pois && n == hk) {
           do {
+            const float J = cq->J[qh][lane];
             if (j.additive) {  // fma(j1, y, j0) = j0 exactly: add_jump's bits, one operation
 #pragma unroll
-              for (int i = 0; i < D; ++i) x[i] = __fmaf_rn(f.j0[i], qJ[0], x[i]);
+              for (int i = 0; i < D; ++i) x[i] = __fmaf_rn(f.j0[i], J, x[i]);
             } else if (f.exact) {
               float y[D];
 #pragma unroll
               for (int i = 0; i < D; ++i) y[i] = x[i];
-              add_jump(f, x, y, qJ[0]);
+              add_jump(f, x, y, J);
             } else {
-              add_jump(f, x, xo, qJ[0]);
+              add_jump(f, x, xo, J);
             }
-            jsum += qJ[0];
-            if constexpr (COUPLED) acc.jump(f, qJ[0]);
-#pragma unroll
-            for (int q = 0; q + 1 < JQ; ++q) {
-              qk[q] = qk[q + 1];
-              qJ[q] = qJ[q + 1];
-            }
-            qk[JQ - 1] = -1;
-          } while (n == qk[0]);
+            jsum += J;
+            if constexpr (COUPLED) acc.jump(f, J);
+            ++qh;
+            hk = qh < CLOCK_JQ ? cq->k[qh][lane] : -1;
+          } while (n == hk);
         }
         if (pois && j.additive && ((jm >> s) & 1u)) {
           // constant jump coefficients: one total per jumping step
@@ -1404,19 +1410,23 @@ __device__ void path_uniform_rng_mode(const KJob& j, const PathCtx& c, bool lane
   bool spill = false;
   for (int32_t n0 = 0; n0 < N; n0 += G * U) {
     if (clock && n0 >= wend) {
+      // the previous window's queue is used up: its jumps all fell before n0
       wend = n0 + win;
-#pragma unroll
-      for (int q = 0; q < JQ; ++q) {
-        if (__any(kj < wend)) {
-          if (kj < wend) {
-            qk[q] = kj;
-            qJ[q] = jump_mark(f, jc.m1, jc.m2);
-            ++jc.j;
-            jc.draw(c.k0, c.k1, plo, phi, f.inv_rate);
-            kj = step_of(jc.tau, j.n_over_T, N);
-          }
+      int32_t tail = 0;
+      for (int q = 0; q < CLOCK_JQ; ++q) {
+        if (!__any(kj < wend)) break;
+        if (kj < wend) {
+          cq->k[q][lane] = kj;
+          cq->J[q][lane] = jump_mark(f, jc.m1, jc.m2);
+          ++jc.j;
+          jc.draw(c.k0, c.k1, plo, phi, f.inv_rate);
+          kj = step_of(jc.tau, j.n_over_T, N);
+          tail = q + 1;
         }
       }
+      if (tail < CLOCK_JQ) cq->k[tail][lane] = -1;
+      qh = 0;
+      hk = cq->k[0][lane];
       spill = __any(kj < wend);
     }
     if (spill) {
