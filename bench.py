@@ -297,7 +297,7 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
 # keyed by the launch shape's chunks per tile: nearly all of it is the chunk
 # partials' writes, which scatter more with one chunk per tile; and the
 # fixed-increment kernel (2^21 x 500).
-TRAFFIC_MIX_ROUND = {1: 1.140e8, 2: 8.23e7}  # tile_chunks -> HBM bytes of one mix round
+TRAFFIC_MIX_ROUND = {1: 8.652e7, 2: 6.248e7}  # tile_chunks -> HBM bytes of one mix round
 TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 # VALU instructions (wavefront-level, SQ_INSTS_VALU summed over the group
 # kernels) of one round of the served mix (128 requests, seed 0) and the fp32

@@ -408,7 +408,10 @@ typedef struct sdemc_batch_dev {
                                     segments, tiles, 0}, read on the device     */
   const int32_t* finalize_dev;   /* [max_jobs] nonzero: finalise slot this launch */
   double* partials_dev;          /* [partial_chunks][2]                         */
-  int32_t* chunk_flags_dev;      /* [partial_chunks] flags                      */
+  int32_t* chunk_flags_dev;      /* [partial_chunks] flags: zero when first
+                                    handed to the library; the group kernels
+                                    write only set flags and the finalisation
+                                    resets those it reads                      */
   double* stats_dev;             /* [max_jobs][4]                               */
   int32_t* counter_dev;          /* [SDEMC_MAX_GROUPS + 1]: the groups' tile
                                     counters, then the launch's fault word; all

@@ -28,10 +28,12 @@ __global__ void k_zero_counter(int32_t* counter, int32_t n) {
 // Fixed-order finalisation of the slots flagged this launch.  A slot that is
 // not prepared or whose chunks lie outside the partial pool gets NaN sums and
 // flag SDEMC_FLAG_TABLE; so does every slot when the launch's fault word is set.
+// The chunk flags it reads are reset to zero (the group kernels write only
+// set flags).
 __global__ __launch_bounds__(256) void k_finalize_slots(const KJob* __restrict__ jobs,
                                                         const int32_t* __restrict__ mask,
                                                         const double* partials,
-                                                        const int32_t* cflags,
+                                                        int32_t* cflags,
                                                         const int32_t* __restrict__ fault,
                                                         int64_t partial_chunks, double* stats) {
   const int j = blockIdx.x;
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(256) void k_finalize_slots(const KJob* __restrict__
     return;
   }
   finalize_block(partials, cflags, job.chunk_base, nc, job.n_paths, st,
-                 *fault ? SDEMC_FLAG_TABLE : 0);
+                 *fault ? SDEMC_FLAG_TABLE : 0, cflags);
 }
 
 static int check_batch(const sdemc_batch_dev* b, const sdemc_batch_tune* tune) {

@@ -54,7 +54,7 @@ __device__ __forceinline__ void chunk_body(const KJob& job, int64_t chunk, doubl
     if (valid) v = job.pay.disc * (payoff_raw<D>(job.pay, xf) - payoff_raw<D>(job.pay, xc));
   }
   const double dv = valid ? (double)v : 0.0;
-  chunk_store(dv, dv * dv, ovf, part, cf);
+  chunk_store<true>(dv, dv * dv, ovf, part, cf);
 }
 
 // Class of a job: dedicated kernels for the 1-D 'diag' and 2-D 'indep'

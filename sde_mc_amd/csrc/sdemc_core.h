@@ -1042,6 +1042,10 @@ __device__ __forceinline__ uint32_t pois_draw_window(const KJob& j, uint32_t k0,
 // ---------------------------------------------------------------------------
 // Chunk reduction: fp64 butterfly over the 64 lanes, lane 0 stores.
 // ---------------------------------------------------------------------------
+// SPARSE_FLAG: the flag is written only when set (the batched launch, whose
+// flag slots start at zero and are cleared again by the finalisation that
+// reads them), so a chunk's store is its 16-byte partial alone.
+template <bool SPARSE_FLAG = false>
 __device__ __forceinline__ void chunk_store(double s, double q, int flag, double* partial2,
                                             int32_t* cflag) {
 #pragma unroll
@@ -1055,14 +1059,17 @@ __device__ __forceinline__ void chunk_store(double s, double q, int flag, double
   if ((threadIdx.x & (WAVE - 1)) == 0) {
     partial2[0] = s;
     partial2[1] = q;
-    cflag[0] = any_flag;
+    if (!SPARSE_FLAG || any_flag != 0) cflag[0] = any_flag;
   }
 }
 
 // Fixed-order reduction of chunk partials into {sum, sumsq, n, flags}: thread
-// t takes chunks t, t+256, ...; then a 256-leaf LDS tree.
+// t takes chunks t, t+256, ...; then a 256-leaf LDS tree.  With `clear` (the
+// batched launch's flag slots, the same array as cflags) every set flag read
+// is reset to zero for the slot's next use.
 __device__ __forceinline__ void finalize_block(const double* partials, const int32_t* cflags, int64_t base,
-                               int64_t nc, uint64_t n, double* st, int extra_flags = 0) {
+                               int64_t nc, uint64_t n, double* st, int extra_flags = 0,
+                               int32_t* clear = nullptr) {
   __shared__ double ss[256];
   __shared__ double qs[256];
   __shared__ int fs[256];
@@ -1072,7 +1079,9 @@ __device__ __forceinline__ void finalize_block(const double* partials, const int
   for (int64_t c = t; c < nc; c += 256) {
     s += partials[2 * (base + c)];
     q += partials[2 * (base + c) + 1];
-    fl |= cflags[base + c];
+    const int32_t cf = cflags[base + c];
+    fl |= cf;
+    if (clear && cf != 0) clear[base + c] = 0;
   }
   ss[t] = s;
   qs[t] = q;

@@ -115,6 +115,24 @@ def test_class_mismatch_is_flagged_not_silent():
         E.check_flags(st[3], 'job')
 
 
+def test_set_chunk_flags_do_not_outlive_their_round():
+    # the group kernels write a chunk's flag only when it is set and the
+    # finalisation resets the flags it reads: a round whose job overflows its
+    # adapted-grid cap is flagged, and the next job on the same partial slots
+    # is clean and equals its plain launch
+    from sde_mc_amd import _lib as L
+    r = BatchRunner(max_jobs=4)
+    bad = J.job_for(J.merton_sde(rate=30.0), 1.0, 20, 4000, grid='adapted', max_steps=22,
+                    option=sm.EuroCall(1.0), seed=5)
+    with pytest.raises(L.SdemcError, match='max_steps'):
+        r.run([bad])
+    good = J.job_for(J.merton_sde(), 1.0, 20, 4000, grid='adapted', option=sm.EuroCall(1.0),
+                     seed=6)
+    again = r.run([good])
+    assert again[0, 3] == 0
+    assert np.array_equal(again[0], np.array(_plain(good)))
+
+
 def test_autotune_picks_a_candidate_and_it_prices_the_same():
     from sde_mc_amd.autotune import autotune
     jobs = _mix(6, seed=11)
