@@ -175,12 +175,16 @@ __device__ __forceinline__ void raise_fault(int32_t* fault, int lane) {
 // Persistent tile kernel of one class group.  hdr = {first segment,
 // segments, tiles}; tile indices come from this group's counter.  U is the
 // step unroll of the uniform single-level path (classes 0 and 4).
-// At least 4 wavefronts per SIMD: the class kernels of one family need at
-// most 64 VGPRs (8 wavefronts); the generic class, whose cases' register
-// needs go up to ~250 VGPRs for the 4-D two-driver family, is held to 128
-// (a few of its cases spill a little) instead of running at 2 per SIMD.
+// Wavefronts per SIMD: the dedicated class kernels are held to 64 VGPRs so
+// that 8 fit (the served mix's per-step count kernel went from 66 VGPRs, 7
+// wavefronts, to 64 with 20 B of spills per lane, and the round from 41.4 to
+// 39.7 ms; the coupled and adapted classes move by -7% .. +2%); the generic
+// class, whose cases' register needs go up to ~250 VGPRs for the 4-D
+// two-driver family, to 128 (4 wavefronts; a few of its cases spill a
+// little) instead of running at 2 per SIMD.
 template <int CLS, int U>
-__global__ __launch_bounds__(512, 4) void k_batch(const KJob* __restrict__ jobs,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(
+    CLS == SDEMC_CLASS_GENERIC ? 4 : 8))) void k_batch(const KJob* __restrict__ jobs,
                                                const sdemc_segment* __restrict__ segs,
                                                const int32_t* __restrict__ hdr,
                                                double* __restrict__ partials,
