@@ -82,6 +82,22 @@ def test_graph_replays_are_consistent():
     assert np.array_equal(r3.run(jobs), first)
 
 
+def test_jump_queue_overflow_and_long_windows_equal_plain():
+    # the arrival clock's LDS queue at its edges, in the batched class kernels
+    # against the plain ones: several jumps per lane and step (most fill
+    # windows overflow the queue and draw in the step), single and coupled
+    # levels, both jump rules; and a low rate whose window spans the horizon
+    jobs = [J.job_for(J.merton_sde(rate=30.0), 1.0, 20, 3000, option=sm.EuroCall(1.0), seed=11),
+            J.job_for(J.merton_sde(rate=30.0), 1.0, 40, 3000, coarse_factor=2,
+                      option=sm.EuroCall(1.0), seed=12, exact_jumps=True),
+            J.job_for(J.levy_rainbow_sde(f=(0.2, 0.2), eps=0.02), 1.0, 1000, 2000,
+                      option=sm.Rainbow(1.0, log=True), seed=13),
+            J.job_for(J.merton_sde(rate=0.05), 0.25, 700, 5000, option=sm.EuroCall(1.0), seed=14)]
+    plain = np.array([_plain(j) for j in jobs])
+    assert np.array_equal(BatchRunner(max_jobs=8).run(jobs), plain)
+    assert np.all(plain[:, 3] == 0)
+
+
 def test_generic_class_with_dedicated_classes():
     # a 3-D model runs in the generic kernel, next to a dedicated 1-D group
     corr = torch.tensor([[1.0, 0.2, 0.1], [0.2, 1.0, 0.3], [0.1, 0.3, 1.0]])
