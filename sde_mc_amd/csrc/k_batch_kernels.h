@@ -89,8 +89,8 @@ inline int32_t kjob_class_of(const KJob& k) {
 
 // The generic class calls each case's chunk body as a function of its own,
 // once per 64-path chunk: the cases are compiled one by one instead of all
-// inlined into one kernel body (minutes of compile time less; the class runs
-// at 2 wavefronts per SIMD either way).
+// inlined into one kernel body (about 1.5 minutes of compile time instead of
+// 15), and the call costs nothing next to a chunk's path work.
 template <int D, int FAM, int GRID, bool COUPLED>
 __device__ __attribute__((noinline)) void chunk_call(const KJob& job, int64_t chunk, double* part,
                                                      int32_t* cf) {
@@ -174,7 +174,7 @@ __device__ __forceinline__ void raise_fault(int32_t* fault, int lane) {
 
 // Persistent tile kernel of one class group.  hdr = {first segment,
 // segments, tiles}; tile indices come from this group's counter.  U is the
-// step unroll of the uniform single-level path (classes 0 and 4).
+// step unroll of the uniform single-level path (classes 0, 4 and 8).
 // Wavefronts per SIMD: the dedicated class kernels are held to 64 VGPRs so
 // that 8 fit (the served mix's per-step count kernel went from 66 VGPRs, 7
 // wavefronts, to 64 with 20 B of spills per lane, and the round from 41.4 to
