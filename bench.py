@@ -290,6 +290,39 @@ def valu_issue_ceiling(dev, iters=8192, reps=3):
     return rec
 
 
+def hbm_read_ceiling(dev, gib=4, reps=5):
+    """Measured streaming-read rate of the device: sdemc_hbm_read_probe_dev
+    over a `gib` GiB buffer (16 times the last-level cache), 16-byte loads at
+    a grid stride, 4 or 8 workgroups of 256 threads per CU, best of `reps`
+    launches each.  The fixed-increment kernel's bandwidth is stated against
+    this as well as against the 8 TB/s datasheet figure."""
+    import ctypes
+    import torch
+    from sde_mc_amd import _lib as L
+    from sde_mc_amd import engine as E
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    nbytes = gib << 30
+    src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
+    out = torch.empty(cus * 8 * 256, dtype=torch.int32, device=dev)
+    lib = L.lib()
+    best = None
+    for bpc in (4, 8):
+        def run():
+            L.check(lib.sdemc_hbm_read_probe_dev(ctypes.c_void_p(src.data_ptr()), nbytes,
+                                                 ctypes.c_void_p(out.data_ptr()), cus * bpc,
+                                                 E.stream_ptr()), 'sdemc_hbm_read_probe_dev')
+        run()
+        torch.cuda.synchronize()
+        ms = min(_time_events(run, 1) for _ in range(reps))
+        if best is None or ms < best[0]:
+            best = (ms, bpc)
+    del src, out
+    torch.cuda.empty_cache()
+    return {"gbs": nbytes / (best[0] * 1e-3) / 1e9, "bytes": nbytes, "kernel_ms": best[0],
+            "blocks_per_cu": best[1],
+            "probe": "sdemc_hbm_read_probe_dev: 16-byte loads at a grid stride, XOR-folded"}
+
+
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # each scaled by the counter's reading on a 1 GiB device copy (fetch x2.0,
 # write x1.0); profiles/hbm_traffic_served_mix_and_fixed_merton.md holds the
@@ -363,7 +396,7 @@ def fixed_increment_leg(dev, reps, P=1 << 21, N=500, seed=0):
             "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
                          "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
                          "traffic": TRAFFIC_FIXED if P == 1 << 21 and N == 500 else None,
-                         "kernel": "k_fixed<1,merton,uniform> + k_finalize_one"},
+                         "kernel": "k_fixed_uniform<1, merton> + k_finalize_one"},
             "price": mean, "se": se, "closed_form": MERTON_CLOSED_FORM}
 
 
@@ -719,7 +752,10 @@ def main():
     extra = {}
     mlmc_jobs = None
     if not args.no_extra:
-        extra["fixed_increment"] = fixed_increment_leg(dev, args.steps)
+        extra["fixed_increment"] = fixed = fixed_increment_leg(dev, args.steps)
+        ceil = hbm_read_ceiling(dev)
+        fixed["read_ceiling"] = ceil
+        fixed["roofline"]["frac_of_read_ceiling"] = fixed["roofline"]["achieved"] / ceil["gbs"]
         extra["equal_ci"] = equal_ci_leg(reqs, tune, dev)
         extra["mlmc"], mlmc_jobs = mlmc_leg(dev)
         extra["control_variates"] = cv_leg(dev)

@@ -470,6 +470,15 @@ int sdemc_batch_plan_destroy(sdemc_batch_plan* plan);
 int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint32_t* out_dev,
                      int64_t n, void* stream);
 
+/* HBM streaming-read probe (no counterpart in the reference; it measures the
+ * read rate the fixed-increment kernels are held against): `blocks`
+ * workgroups of 256 threads read `bytes` (a multiple of 16, src 16-byte
+ * aligned) from src_dev as 16-byte words at a grid stride, each thread
+ * XOR-folding its words into one uint32 of out_dev[blocks * 256] (so the XOR
+ * of out_dev is the XOR of every 32-bit word of the source). */
+int sdemc_hbm_read_probe_dev(const void* src_dev, int64_t bytes, uint32_t* out_dev,
+                             int32_t blocks, void* stream);
+
 /* VALU issue-rate probes (no counterpart in the reference; they measure the
  * ceilings the path kernels' roofline is priced at): `blocks` workgroups of
  * 256 threads each run `iters` iterations of 32 independent operations of one

@@ -163,6 +163,9 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "sdemc_valu_probe_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_void_p]),
+    "sdemc_hbm_read_probe_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64,
+                                                ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.c_void_p]),
     "sdemc_normals_dev": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
                                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
 }

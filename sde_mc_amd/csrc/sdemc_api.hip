@@ -192,6 +192,21 @@ __global__ void k_normals(uint32_t k0, uint32_t k1, uint64_t path_offset, int64_
 //           parallelism)
 // KIND 0 measures the ceiling the path kernels' VALU instruction counts are
 // priced at; the others the costs of Philox (1, 2, 4) and Box-Muller (3, 5).
+// HBM streaming-read probe: each thread XOR-folds 16-byte words taken at a
+// grid stride and writes one word; over a buffer far larger than the
+// last-level cache this measures the read rate that the fixed-increment
+// kernels are held against.
+__global__ __launch_bounds__(256) void k_hbm_read_probe(const uint4* __restrict__ src, int64_t n,
+                                                        uint32_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t acc = 0u;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint4 v = src[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int32_t iters,
                                                     float a, float b) {
@@ -489,6 +504,16 @@ int sdemc_philox_dev(const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, uint
   if (blocks > 0x7fffffffLL) return SDEMC_ERR_ARG;
   hipLaunchKernelGGL(k_philox, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, ctr_dev,
                      key0, key1, out_dev, n);
+  return hip_status(hipGetLastError());
+}
+
+int sdemc_hbm_read_probe_dev(const void* src_dev, int64_t bytes, uint32_t* out_dev,
+                             int32_t blocks, void* stream) {
+  if (!src_dev || !out_dev || bytes < 16 || (bytes & 15) != 0 ||
+      ((uintptr_t)src_dev & 15) != 0 || blocks < 1)
+    return SDEMC_ERR_ARG;
+  hipLaunchKernelGGL(k_hbm_read_probe, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const uint4*)src_dev, bytes / 16, out_dev);
   return hip_status(hipGetLastError());
 }
 

@@ -221,3 +221,15 @@ def test_noalloc_entry_point_refuses_before_touching_the_device():
     job.steps = 0
     assert lib.sdemc_simulate_noalloc(ctypes.byref(job), ctypes.c_void_p(8), None, 4, None) == \
         L.ERR_ARG
+
+
+def test_hbm_read_probe_refuses_bad_arguments():
+    # null or misaligned source, a size that is not whole 16-byte words, no
+    # workgroups: refused on the host, nothing launched
+    lib = L.load()
+    out = ctypes.c_void_p(64)
+    assert lib.sdemc_hbm_read_probe_dev(None, 1024, out, 4, None) == L.ERR_ARG
+    assert lib.sdemc_hbm_read_probe_dev(ctypes.c_void_p(16), 1000, out, 4, None) == L.ERR_ARG
+    assert lib.sdemc_hbm_read_probe_dev(ctypes.c_void_p(8), 1024, out, 4, None) == L.ERR_ARG
+    assert lib.sdemc_hbm_read_probe_dev(ctypes.c_void_p(16), 1024, out, 0, None) == L.ERR_ARG
+    assert lib.sdemc_hbm_read_probe_dev(ctypes.c_void_p(16), 1024, None, 4, None) == L.ERR_ARG

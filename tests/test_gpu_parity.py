@@ -83,6 +83,22 @@ def test_device_philox_known_answers():
         assert got == want
 
 
+def test_hbm_read_probe_reads_every_word():
+    # the XOR of the probe's outputs is the XOR of every 32-bit word of the
+    # source, at a size whose last grid-stride pass is ragged
+    dev = _dev()
+    words = np.random.default_rng(5).integers(0, 1 << 32, size=(1 << 18) + 4 * 37,
+                                              dtype=np.uint32)
+    src = torch.from_numpy(words.view(np.int32)).to(dev)
+    blocks = 7
+    out = torch.empty(blocks * 256, dtype=torch.int32, device=dev)
+    L.check(L.lib().sdemc_hbm_read_probe_dev(ctypes.c_void_p(src.data_ptr()), words.nbytes,
+                                             ctypes.c_void_p(out.data_ptr()), blocks,
+                                             E.stream_ptr()), 'hbm read probe')
+    got = np.bitwise_xor.reduce(out.cpu().numpy().view(np.uint32))
+    assert got == np.bitwise_xor.reduce(words)
+
+
 def test_device_normals_match_oracle():
     dev = _dev()
     P, n = 256, 64
