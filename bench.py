@@ -331,7 +331,7 @@ def hbm_read_ceiling(dev, gib=4, reps=5):
 # partials' writes, which scatter more with one chunk per tile; and the
 # fixed-increment kernel (2^21 x 500).
 TRAFFIC_MIX_ROUND = {1: 9.504e7, 2: 7.103e7}  # tile_chunks -> HBM bytes of one mix round
-TRAFFIC_FIXED = 5.558e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
+TRAFFIC_FIXED = 5.711e9      # HBM bytes of one fixed-increment launch (2^21 x 500)
 # VALU instructions (wavefront-level, SQ_INSTS_VALU summed over the group
 # kernels) of one round of the served mix (128 requests, seed 0) and the fp32
 # transcendentals among them (SQ_INSTS_VALU_TRANS_F32), from rocprofv3 --pmc;
@@ -396,7 +396,7 @@ def fixed_increment_leg(dev, reps, P=1 << 21, N=500, seed=0):
             "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
                          "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
                          "traffic": TRAFFIC_FIXED if P == 1 << 21 and N == 500 else None,
-                         "kernel": "k_fixed_uniform<1, merton> + k_finalize_one"},
+                         "kernel": "k_fixed_uniform4<1, merton> + k_finalize_one"},
             "price": mean, "se": se, "closed_form": MERTON_CLOSED_FORM}
 
 
@@ -668,7 +668,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.fixed_only:
-        print(json.dumps({"fixed_increment": fixed_increment_leg(dev, args.steps)}))
+        fixed = fixed_increment_leg(dev, args.steps)
+        fixed["read_ceiling"] = ceil = hbm_read_ceiling(dev)
+        fixed["roofline"]["frac_of_read_ceiling"] = fixed["roofline"]["achieved"] / ceil["gbs"]
+        print(json.dumps({"fixed_increment": fixed}))
         return
     if args.cv_only:
         print(json.dumps({"control_variates": cv_leg(dev, args.steps)}))

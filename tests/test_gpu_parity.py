@@ -227,6 +227,63 @@ def test_fixed_benchmarked_levy_model():
     assert abs(s[0] / P - opay.mean()) < 1e-5
 
 
+def _fixed_struct_shifted(fx, dev, shift_dw, shift_counts):
+    """Device inputs of fx with dw placed `shift_dw` floats and counts
+    `shift_counts` bytes past the start of an allocation: off the 16-byte /
+    4-byte alignment the four-paths-per-lane kernel needs."""
+    fin = fx.device_struct(dev)
+    keep = []
+    if shift_dw:
+        dw = fx.t['dw'].reshape(-1)
+        buf = torch.empty(dw.numel() + shift_dw, dtype=torch.float32, device=dev)
+        buf[shift_dw:] = dw
+        fin.dw_dev = buf.data_ptr() + 4 * shift_dw
+        keep.append(buf)
+    if shift_counts:
+        c = fx.t['counts'].reshape(-1)
+        buf = torch.empty(c.numel() + shift_counts, dtype=torch.uint8, device=dev)
+        buf[shift_counts:] = c
+        fin.counts_dev = buf.data_ptr() + shift_counts
+        keep.append(buf)
+    return fin, keep
+
+
+@pytest.mark.parametrize("model,d", [('merton', 1), ('merton', 3), ('levy', 2)])
+def test_fixed_four_paths_per_lane_equals_one_path_per_lane(model, d):
+    # P % 4 == 0 with aligned rows runs the four-paths-per-lane kernel
+    # (16-byte row loads); the same inputs placed off that alignment run the
+    # one-path-per-lane kernel: every path, payoff and the job's sums agree
+    # bit for bit (ragged last wavefront, jumps on)
+    dev = _dev()
+    P, N = 4 * (1024 + 3 * 64 + 5), 67
+    if model == 'merton':
+        sde, opt = J.merton_sde(rate=4.0, dim=d), EuroCall(1.0)
+    else:
+        sde, opt = J.levy_rainbow_sde(), Rainbow(1.0, log=True, discount=math.exp(0.02))
+    job = J.job_for(sde, 1.0, N, P, option=opt, disc=math.exp(-0.02))
+    fx = J.fixed_uniform(job, seed=21)
+    assert int(fx.a['counts'].sum()) > 0
+    res = []
+    for shift in ((0, 0), (1, 0), (0, 1)):
+        print('fixed job P=%d d=%d shift=%s' % (P, d, shift), flush=True)
+        fin, keep = _fixed_struct_shifted(fx, dev, *shift)
+        pay = torch.empty(P, dtype=torch.float32, device=dev)
+        term = torch.empty((1, d, P), dtype=torch.float32, device=dev)
+        ns = torch.empty(P, dtype=torch.int32, device=dev)
+        out = L.PathOut()
+        out.payoff_dev, out.terminal_dev, out.nsteps_dev = (pay.data_ptr(), term.data_ptr(),
+                                                            ns.data_ptr())
+        stats = E.simulate_fixed(job, fin, dev, out=out)
+        res.append((pay.cpu().numpy(), term.cpu().numpy(), ns.cpu().numpy(),
+                    stats.cpu().numpy()))
+    assert res[0][3][3] == 0 and res[0][3][2] == P
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(a, b)
+    opay, oterm, _ = oracle.simulate_fixed(job, fx.host_struct(), P)
+    assert _close(res[0][1], oterm) < TOL
+
+
 def test_fixed_jump_reads_outside_the_marks_raise():
     # counts that claim more marks than jump_size holds: the kernel refuses
     # the reads past n_marks (in bounds, mark 0) and flags the job, which the
