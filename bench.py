@@ -734,11 +734,12 @@ def main():
     if not args.no_serve:
         from sde_mc_amd.serve import PricingServer, latency_summary, serve_plain
         latency = {}
-        # 50, 200, 1000 req/s: the workload's rates; 1500 req/s offers more
-        # work than the plain per-request form can finish (~2.8e11
-        # path-steps/s) but less than the batched rounds can
-        for rate in (50, 200, 1000, 1500):
-            n_req = args.requests * (4 if rate == 1500 else 1)   # long enough to queue up
+        # 50, 200, 1000 req/s: the workload's rates; 1500 and 2500 req/s
+        # offer 3e11 and 5e11 path-steps/s, near and past what the plain
+        # per-request form (kernel, read-back and host work per request)
+        # finishes, and less than the batched rounds can
+        for rate in (50, 200, 1000, 1500, 2500):
+            n_req = args.requests * (4 if rate >= 1500 else 1)   # long enough to queue up
             arr = W.request_mix(n_req, seed=args.seed + rate, rate=rate)
             srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20, tune=tune)
             res = srv.serve(arr)
