@@ -732,6 +732,7 @@ def main():
 
     latency = None
     if not args.no_serve:
+        from sde_mc_amd.batch import job_class
         from sde_mc_amd.serve import PricingServer, latency_summary, serve_plain
         latency = {}
         # 50, 200, 1000 req/s: the workload's rates; 1500 and 2500 req/s
@@ -742,6 +743,8 @@ def main():
             n_req = args.requests * (4 if rate >= 1500 else 1)   # long enough to queue up
             arr = W.request_mix(n_req, seed=args.seed + rate, rate=rate)
             srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20, tune=tune)
+            # server start-up: the round graphs of the stream's kernel classes
+            srv.warm({job_class(W.request_job(r)) for r in arr})
             res = srv.serve(arr)
             served = latency_summary(res)
             served["rounds"] = srv.rounds

@@ -20,6 +20,7 @@ fixed per job (64-path chunks, then chunks in order), so a job alone and in
 any mix gives the same bits.
 """
 import ctypes
+import itertools
 
 import numpy as np
 import torch
@@ -349,20 +350,40 @@ class BatchRunner:
         groups = tuple(int(g) for g in groups) or (L.CLASS_GENERIC,)
         stream = E.stream_ptr()
         if self.use_graph:
-            key = (id(buf), groups, tune.key(), self.partials.data_ptr())
-            plan = self._plans.get(key)
-            if plan is None:
-                h = ctypes.c_void_p()
-                L.check(self.lib.sdemc_batch_plan_create(
-                    ctypes.byref(self.batch_dev(buf, groups)), ctypes.byref(tune.c()),
-                    ctypes.byref(h)), 'sdemc_batch_plan_create')
-                plan = h
-                self._plans[key] = plan
+            plan = self._graph(buf, groups, tune)
             L.check(self.lib.sdemc_batch_plan_launch(plan, stream), 'sdemc_batch_plan_launch')
         else:
             L.check(self.lib.sdemc_batch_launch(ctypes.byref(self.batch_dev(buf, groups)),
                                                 ctypes.byref(tune.c()), stream),
                     'sdemc_batch_launch')
+
+    def _graph(self, buf, groups, tune):
+        """The captured graph of a round in `buf` with these class groups and
+        launch shape, captured on first use."""
+        key = (id(buf), groups, tune.key(), self.partials.data_ptr())
+        plan = self._plans.get(key)
+        if plan is None:
+            h = ctypes.c_void_p()
+            L.check(self.lib.sdemc_batch_plan_create(
+                ctypes.byref(self.batch_dev(buf, groups)), ctypes.byref(tune.c()),
+                ctypes.byref(h)), 'sdemc_batch_plan_create')
+            plan = h
+            self._plans[key] = plan
+        return plan
+
+    def warm(self, classes, tune=None):
+        """Capture the graph of every round the given kernel classes can form
+        (each ordered selection of them, as a round lists its groups, on
+        every round buffer), so that no later round pays for a capture: a
+        serving loop's start-up.  Returns the number of graphs held."""
+        if self.use_graph:
+            tune = tune or self.tune
+            cls = sorted(set(int(c) for c in classes))
+            for r in range(1, min(len(cls), L.MAX_GROUPS) + 1):
+                for groups in itertools.permutations(cls, r):
+                    for buf in self.bufs:
+                        self._graph(buf, groups, tune)
+        return len(self._plans)
 
     def read_stats(self, buf, slots):
         """Queue the D2H copy of the finalised slots' statistics into the

@@ -105,3 +105,24 @@ def test_poisson_stream_completes_with_latencies():
     assert srv.rounds >= 2
     # Merton requests price within 4 se + the Euler bias of the closed form
     assert _check_merton_prices(reqs, res) >= 10
+
+
+def test_warmed_server_captures_no_graph_while_serving():
+    # warming with the stream's kernel classes captures every round graph
+    # the stream can use (each ordered selection of its classes, both round
+    # buffers); serving then adds none, and the results are the unwarmed
+    # server's bit for bit
+    from sde_mc_amd.batch import job_class
+    reqs = _small_mix(40, seed=12, rate=800)
+    classes = {job_class(W.request_job(r)) for r in reqs}
+    print('warm classes %s' % sorted(classes), flush=True)
+    srv = PricingServer(max_slots=64, hbm_budget_bytes=64 << 20)
+    n = srv.warm(classes)
+    k = len(classes)
+    assert n == 2 * sum(math.perm(k, r) for r in range(1, k + 1))
+    res = srv.serve(_small_mix(40, seed=12, rate=800))
+    assert len(srv.runner._plans) == n
+    cold = PricingServer(max_slots=64, hbm_budget_bytes=64 << 20).serve(
+        _small_mix(40, seed=12, rate=800), realtime=False)
+    for r in reqs:
+        assert np.array_equal(np.array(res[r.rid].stats[:3]), np.array(cold[r.rid].stats[:3]))
