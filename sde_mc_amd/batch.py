@@ -45,9 +45,15 @@ def noise_width(model):
 
 # Issue cost (VALU slots per wavefront step) of one normal and of a
 # wavefront-wide jump draw, fitted to the measured per-model throughput of the
-# plain kernels (1-D Merton at 500 steps and the 2-D Levy rainbow at 200).
+# plain kernels (1-D Merton at 500 steps and the 2-D Levy rainbow at 200);
+# the uniform grid's arrival clock, for either jump law, at the time of the
+# served mix's 2-D Levy clock-mode kernel (its cost per unit then equals the
+# Merton and per-step-count kernels', so a round's cost predicts its time
+# whatever its mix of classes).
 SLOTS_PER_NORMAL = 60.0
 SLOTS_PER_JUMP_STEP = {L.JUMP_LOGNORMAL: 280.0, L.JUMP_LEVY: 475.0}
+SLOTS_PER_CLOCK_JUMP_STEP = 280.0
+POIS_MIN_MU = 1.0 / 16.0  # the uniform grid's Levy jumps switch to per-step counts here
 
 
 def job_cost(job):
@@ -55,9 +61,12 @@ def job_cost(job):
     slots: per step the normals drawn plus the jump work a wavefront sees (a
     jump draw whenever some lane jumps), x(1 + 1/M) for coupled pairs."""
     m = job.model
-    wave_jumps = min(1.0, 64.0 * m.rate * job.T / job.steps)
-    c = job.steps * (SLOTS_PER_NORMAL * noise_width(m)
-                     + SLOTS_PER_JUMP_STEP.get(m.jump_law, 300.0) * wave_jumps)
+    mu = m.rate * job.T / job.steps
+    wave_jumps = min(1.0, 64.0 * mu)
+    per_jump = SLOTS_PER_JUMP_STEP.get(m.jump_law, 300.0)
+    if job.grid == L.GRID_UNIFORM and not (m.jump_law == L.JUMP_LEVY and mu >= POIS_MIN_MU):
+        per_jump = SLOTS_PER_CLOCK_JUMP_STEP
+    c = job.steps * (SLOTS_PER_NORMAL * noise_width(m) + per_jump * wave_jumps)
     if job.coarse_factor:
         c *= 1.0 + 1.0 / job.coarse_factor
     return c

@@ -104,11 +104,10 @@ class PricingServer:
     budget lasts; 'fair' gives the oldest request half the budget, the next
     half of the rest, and so on.  A round that holds a single request from its
     first chunk to its last runs that request's plain kernel instead (the
-    same bits, less launch overhead), and a request that is alone -- the only
-    one with work left and none waiting for admission -- takes all its
-    remaining chunks in one round whatever the budget (its plain kernel if it
-    has not started): split into rounds it would only add host gaps while
-    nothing else needs the device."""
+    same bits, less launch overhead).  Rounds stay within the budget even for
+    a request that is alone: run whole, a large one would hold every request
+    that arrives meanwhile behind it, a backlog that a server near its
+    capacity does not work off."""
 
     def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=4.0, tune=None,
                  grid=L.GRID_UNIFORM, job_builder=None, policy='srpt', plain_single=True,
@@ -137,6 +136,7 @@ class PricingServer:
         self.grid = grid
         self.job_builder = job_builder or (lambda r: request_job(r, grid))
         self.in_flight = deque()
+        self._last_done = None
         self.results = {}
         self.peak_used = 0
         self.rounds = 0
@@ -200,10 +200,6 @@ class PricingServer:
         todo = [a for a in self.active.values() if a.next < a.nc]
         if not todo:
             return None
-        if self.plain_single and len(todo) == 1 and not self.queue:
-            a = todo[0]
-            first, a.next = a.next, a.nc
-            return [a.slot], [first], [a.nc - first], [a.cost], [a.slot]
         budget = self.cost_rate * self.round_s
         take = {a.slot: 0 for a in todo}
         left = budget
@@ -257,6 +253,10 @@ class PricingServer:
         start = torch.cuda.Event(enable_timing=True)
         cost = float(np.dot(counts, costs))
         full = sum(counts) >= self.fill_chunks
+        # rounds that fill the device and mix requests refine the rate that
+        # sizes rounds; a round of one request (of one model, whose speed per
+        # cost unit is not the mix's) would swing it
+        rated = cost if full and len(slots) > 1 else 0.0
         if (self.plain_single and len(slots) == 1 and firsts[0] == 0
                 and final == slots):
             # one whole request: its plain kernel gives the same bits
@@ -276,8 +276,7 @@ class PricingServer:
             host.copy_(stats, non_blocking=True)
             done = torch.cuda.Event(enable_timing=True)
             done.record()
-            self.in_flight.append(('plain', (host, done, slots[0]), final, start,
-                                   cost if full else 0.0))
+            self.in_flight.append(('plain', (host, done, slots[0]), final, start, rated))
         else:
             self._to_table(slots)
             buf = next(b for b in self.runner.bufs
@@ -288,7 +287,7 @@ class PricingServer:
             start.record()
             self.runner.launch(buf, rp.groups)
             self.runner.read_stats(buf, final)
-            self.in_flight.append(('batch', buf, final, start, cost if full else 0.0))
+            self.in_flight.append(('batch', buf, final, start, rated))
         self.rounds += 1
 
     def _host_stats(self):
@@ -313,8 +312,17 @@ class PricingServer:
             block = False
             self.in_flight.popleft()
             ms = start.elapsed_time(done)
+            prev, self._last_done = self._last_done, done
+            if prev is not None:
+                # a round queued behind the previous one starts when that ends:
+                # its time is its own, not the wait (with the wait counted,
+                # large rounds would look faster per cost unit than small ones
+                # and the budget would feed on itself)
+                ms -= max(0.0, start.elapsed_time(prev))
             if cost > 0 and ms > 0.05:  # rounds that filled the device refine the rate
-                self.cost_rate = 0.7 * self.cost_rate + 0.3 * (cost / (ms * 1e-3))
+                # one round moves the estimate by at most a factor of two
+                r = min(max(cost / (ms * 1e-3), 0.5 * self.cost_rate), 2.0 * self.cost_rate)
+                self.cost_rate = 0.7 * self.cost_rate + 0.3 * r
             now = self._now()
             for s in final:
                 if kind == 'plain':
