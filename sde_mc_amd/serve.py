@@ -111,7 +111,7 @@ class PricingServer:
 
     def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=4.0, tune=None,
                  grid=L.GRID_UNIFORM, job_builder=None, policy='srpt', plain_single=True,
-                 tile_chunks=1):
+                 tile_chunks=1, rate_cap=2.5e12):
         if policy not in ('srpt', 'fair'):
             raise ValueError("policy must be 'srpt' or 'fair'")
         self.policy = policy
@@ -127,7 +127,10 @@ class PricingServer:
         self.queue = deque()
         self.active = {}
         self.round_s = round_ms * 1e-3
-        self.cost_rate = 1.2e12     # cost units (VALU slots) per second, refined per round
+        self.cost_rate = 2.0e12     # cost units (VALU slots) per second, refined per round
+        # upper bound of the refined rate: the served mix settles at 2.0-2.8e12
+        # on MI355X, and an overestimate makes every round longer
+        self.rate_cap = rate_cap
         # a round never holds fewer chunks than fill the device twice over
         # (unless less work is left): smaller rounds leave CUs idle
         t = self.runner.tune
@@ -253,10 +256,8 @@ class PricingServer:
         start = torch.cuda.Event(enable_timing=True)
         cost = float(np.dot(counts, costs))
         full = sum(counts) >= self.fill_chunks
-        # rounds that fill the device and mix requests refine the rate that
-        # sizes rounds; a round of one request (of one model, whose speed per
-        # cost unit is not the mix's) would swing it
-        rated = cost if full and len(slots) > 1 else 0.0
+        # rounds that fill the device refine the rate that sizes rounds
+        rated = cost if full else 0.0
         if (self.plain_single and len(slots) == 1 and firsts[0] == 0
                 and final == slots):
             # one whole request: its plain kernel gives the same bits
@@ -315,14 +316,12 @@ class PricingServer:
             prev, self._last_done = self._last_done, done
             if prev is not None:
                 # a round queued behind the previous one starts when that ends:
-                # its time is its own, not the wait (with the wait counted,
-                # large rounds would look faster per cost unit than small ones
-                # and the budget would feed on itself)
+                # its time is its own, not the wait
                 ms -= max(0.0, start.elapsed_time(prev))
             if cost > 0 and ms > 0.05:  # rounds that filled the device refine the rate
                 # one round moves the estimate by at most a factor of two
                 r = min(max(cost / (ms * 1e-3), 0.5 * self.cost_rate), 2.0 * self.cost_rate)
-                self.cost_rate = 0.7 * self.cost_rate + 0.3 * r
+                self.cost_rate = min(0.7 * self.cost_rate + 0.3 * r, self.rate_cap)
             now = self._now()
             for s in final:
                 if kind == 'plain':
