@@ -308,14 +308,14 @@ class BatchRunner:
             self.jobs.copy_(self.h_jobs, non_blocking=True)
             return
         slots = sorted(int(s) for s in slots)
-        i = 0
-        while i < len(slots):  # coalesce runs of consecutive slots
-            j = i
-            while j + 1 < len(slots) and slots[j + 1] == slots[j] + 1:
-                j += 1
-            a, b = slots[i] * self.kb, (slots[j] + 1) * self.kb
-            self.jobs[a:b].copy_(self.h_jobs[a:b], non_blocking=True)
-            i = j + 1
+        if not slots:
+            return
+        # one copy of the span: the copies are ordered on the stream after the
+        # rounds already queued, and slots inside the span that are not
+        # re-staged get the bytes they already hold, so a round's host cost
+        # does not grow with the number of its new requests
+        a, b = slots[0] * self.kb, (slots[-1] + 1) * self.kb
+        self.jobs[a:b].copy_(self.h_jobs[a:b], non_blocking=True)
 
     def check_tables(self, buf, groups):
         """sdemc_batch_check of the staged round against the host copy of the
@@ -397,8 +397,12 @@ class BatchRunner:
     def read_stats(self, buf, slots):
         """Queue the D2H copy of the finalised slots' statistics into the
         round's pinned buffer and record its completion event."""
-        for s in slots:
-            buf.h_stats[4 * s:4 * s + 4].copy_(self.stats[4 * s:4 * s + 4], non_blocking=True)
+        slots = [int(s) for s in slots]
+        if slots:
+            # one copy of the span of the finalised slots (the others' rows
+            # come along and are not read)
+            a, b = 4 * min(slots), 4 * (max(slots) + 1)
+            buf.h_stats[a:b].copy_(self.stats[a:b], non_blocking=True)
         buf.done = torch.cuda.Event(enable_timing=True)
         buf.done.record()
 
