@@ -735,10 +735,11 @@ def main():
         from sde_mc_amd.batch import job_class
         from sde_mc_amd.serve import PricingServer, latency_summary, serve_plain
         latency = {}
-        # 50, 200, 1000 req/s: the workload's rates; 1500 req/s offers 3.0e11
-        # path-steps/s, near what the plain per-request form (kernel,
-        # read-back and host work per request) finishes
-        for rate in (50, 200, 1000, 1500):
+        # 50, 200, 1000 req/s: the workload's rates; 1500, 2000 and 2500 req/s
+        # offer 3.0e11, 4.4e11 and 5.0e11 path-steps/s: up to and past what
+        # the plain per-request form (kernel, read-back and host work per
+        # request) finishes, and up to what the batched rounds can
+        for rate in (50, 200, 1000, 1500, 2000, 2500):
             n_req = args.requests * (4 if rate >= 1500 else 1)   # long enough to queue up
             arr = W.request_mix(n_req, seed=args.seed + rate, rate=rate)
             srv = PricingServer(max_slots=256, hbm_budget_bytes=512 << 20, tune=tune)

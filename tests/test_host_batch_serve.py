@@ -284,3 +284,26 @@ def test_plan_splits_work_by_the_round_budget_shortest_first():
     srv.fill_chunks = 30
     srv.active = {0: big}
     assert srv._plan() == ([0], [38], [30], [50.0], []) and big.next == 68
+
+
+def test_serving_loops_pause_the_cyclic_collector():
+    # a full collection stops the host for tens of milliseconds; the serving
+    # loops run with the cyclic collector off and restore it after, also
+    # when the loop raises
+    import gc
+    from sde_mc_amd.serve import no_cyclic_gc
+    assert gc.isenabled()
+    with no_cyclic_gc():
+        assert not gc.isenabled()
+    assert gc.isenabled()
+    with pytest.raises(RuntimeError):
+        with no_cyclic_gc():
+            raise RuntimeError('x')
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with no_cyclic_gc():
+            pass
+        assert not gc.isenabled()  # left as found
+    finally:
+        gc.enable()
