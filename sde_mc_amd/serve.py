@@ -123,10 +123,11 @@ class PricingServer:
     budget lasts; 'fair' gives the oldest request half the budget, the next
     half of the rest, and so on.  A round that holds a single request from its
     first chunk to its last runs that request's plain kernel instead (the
-    same bits, less launch overhead).  Rounds stay within the budget even for
-    a request that is alone: run whole, a large one would hold every request
-    that arrives meanwhile behind it, a backlog that a server near its
-    capacity does not work off."""
+    same bits, less launch overhead), and a request that is alone -- the only
+    one with work left and none waiting for admission -- takes all its
+    remaining chunks in one round whatever the budget (its plain kernel if it
+    has not started): split into rounds it would only add host gaps while
+    nothing else needs the device."""
 
     def __init__(self, max_slots=256, hbm_budget_bytes=256 << 20, round_ms=4.0, tune=None,
                  grid=L.GRID_UNIFORM, job_builder=None, policy='srpt', plain_single=True,
@@ -222,6 +223,10 @@ class PricingServer:
         todo = [a for a in self.active.values() if a.next < a.nc]
         if not todo:
             return None
+        if self.plain_single and len(todo) == 1 and not self.queue:
+            a = todo[0]
+            first, a.next = a.next, a.nc
+            return [a.slot], [first], [a.nc - first], [a.cost], [a.slot]
         budget = self.cost_rate * self.round_s
         take = {a.slot: 0 for a in todo}
         left = budget

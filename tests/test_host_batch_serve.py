@@ -258,10 +258,10 @@ def test_autotune_cache_key_follows_the_build(monkeypatch, tmp_path):
     assert A._build_key([BatchTune(2, 16, 1, 0, 1)]) != A._build_key([BatchTune(2, 16, 1, 0, 2)])
 
 
-def test_plan_splits_work_by_the_round_budget_shortest_first():
-    # the round budget (cost_rate x round_s = 1000 cost units) cuts a large
-    # request into rounds, alone or not; the request with the least work
-    # left goes first and is finalised in the round that takes its last chunk
+def test_plan_runs_a_lone_request_whole_and_splits_shared_work():
+    # the round budget (cost_rate x round_s = 1000 cost units) splits work
+    # only while something else needs the device: a request alone, with none
+    # waiting for admission, takes all its remaining chunks in one round
     from collections import deque
     from sde_mc_amd.serve import PricingServer, _Active
     srv = object.__new__(PricingServer)
@@ -275,15 +275,23 @@ def test_plan_splits_work_by_the_round_budget_shortest_first():
 
     big = active(0, 100, 50.0)  # 5000 units
     srv.active = {0: big}
-    assert srv._plan() == ([0], [0], [20], [50.0], []) and big.next == 20
+    assert srv._plan() == ([0], [0], [100], [50.0], [0]) and big.next == 100
+    # with a second request the shortest goes first and the budget cuts the rest
+    big.next = 0
     small = active(1, 2, 50.0)
     srv.active = {0: big, 1: small}
     slots, firsts, counts, _, final = srv._plan()
-    assert (slots, firsts, counts, final) == ([1, 0], [0, 20], [2, 18], [1]) and big.next == 38
-    # a round below the device fill is topped up in the same order
-    srv.fill_chunks = 30
+    assert (slots, firsts, counts, final) == ([1, 0], [0, 0], [2, 18], [1]) and big.next == 18
+    # alone again after a shared round: the rest in one round
     srv.active = {0: big}
-    assert srv._plan() == ([0], [38], [30], [50.0], []) and big.next == 68
+    assert srv._plan() == ([0], [18], [82], [50.0], [0])
+    # a request waiting for admission keeps the budget in force, and a round
+    # below the device fill is topped up in the same order
+    big.next = 0
+    srv.queue.append(object())
+    assert srv._plan() == ([0], [0], [20], [50.0], [])
+    srv.fill_chunks = 30
+    assert srv._plan() == ([0], [20], [30], [50.0], [])
 
 
 def test_serving_loops_pause_the_cyclic_collector():
